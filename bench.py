@@ -1,0 +1,162 @@
+#!/usr/bin/env python
+"""Throughput bench of the dexel dilation hot path on one MI355X.
+
+Workload (one step): dilation by r=16 of a direct dexel volume on a
+1024x1024 grid, z-extent 1024, Poisson(6) segments per column with
+exponential (mean 40 dexel) gaps and lengths, endpoints stored as FP32 on the
+device (predicates and endpoint arithmetic in FP64).  Inputs are resident in
+HBM before the timed region; a step is one vo_morph_device() call (pass 1,
+pass 2, output CSR assembly), synchronised.
+
+Prints ONE JSON line (see the keys below).  `roofline` is computed for the
+dominant kernel from its average duration over the timed steps, measured with
+HIP events that the library records on its own stream around each kernel;
+its byte count is algorithmic: per column pass 1 reads (2*s_in*E + 4) and
+writes (3*m*E + 4), pass 2 reads (3*m*E + 4) and writes (2*s_out*E + 4),
+with E = 4 bytes and m the oracle's pass-1 triple count for this input.
+`cpu_baseline` times the CPU oracle (the FP64 restatement of `ours`) on a
+bounded crop of the same volume, on this host's cores.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "dexel columns/s + % HBM peak, 1024² r=16 dilation; batched FP32 vs plain FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(vol, radius, threads, crop):
+    """Oracle `ours` on a crop x crop window of the workload; also returns the
+    full-volume mean pass-1 triple count m (needed for the byte figure)."""
+    import oracle
+    from voroffset_amd.volume import CompressedVolume
+
+    t = time.perf_counter()
+    po = oracle.pass1(vol, radius, threads=threads)[0]
+    m = float(np.diff(po).mean())
+    log(f"oracle pass-1 count m={m:.3f} ({time.perf_counter() - t:.1f} s)")
+    cols = [vol.at(x, y) for y in range(crop) for x in range(crop)]
+    sample = CompressedVolume.from_columns(crop, crop, cols, origin=vol.origin, extent=vol.extent,
+                                           spacing=vol.spacing, padding=vol.padding)
+    t = time.perf_counter()
+    oracle.morph(sample, "dilation", radius, "ours", threads=threads)
+    dt = time.perf_counter() - t
+    return m, {"value": sample.ncols / dt, "unit": "dexel columns/s", "cores": threads, "kind": "port",
+               "sample": f"{crop}x{crop} crop of the same volume, oracle `ours` (FP64) dilation r={radius:g}, "
+                         f"{threads} threads, {dt:.1f} s"}
+
+
+def run_device(ctx, vol, radius, precision, steps, warmup):
+    import torch
+
+    zlo, zhi = vol.z_range()
+    shift = float(np.round(0.5 * (zlo + zhi))) if precision == "fp32" else 0.0
+    dtype = np.float32 if precision == "fp32" else np.float64
+    d_off = torch.from_numpy(vol.off).cuda()
+    d_z = torch.from_numpy((vol.z - shift).astype(dtype)).cuda()
+    torch.cuda.synchronize()
+    args = ("dilation", precision, radius, vol.nx, vol.ny, d_off.data_ptr(), d_z.data_ptr(), zlo - shift, zhi - shift)
+    for _ in range(warmup):
+        ctx.morph_device(*args)
+    torch.cuda.synchronize()
+    p1 = p2 = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, count, (t1, t2) = ctx.morph_device(*args)
+        p1 += t1
+        p2 += t2
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return dt / steps, p1 / steps, p2 / steps, count
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--side", type=int, default=1024)
+    ap.add_argument("--radius", type=float, default=16.0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-crop", type=int, default=384)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (this build has no multi-GPU path); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from voroffset_amd import _native, synth
+
+    R, side = args.radius, args.side
+    vol = synth.poisson_columns(side, side, lam=6, zmax=1024, seed=1234)
+    ncols = vol.ncols
+    ctx = _native.Context(0)
+
+    step_s, p1_ms, p2_ms, count = run_device(ctx, vol, R, "fp32", args.steps, args.warmup)
+    log(f"fp32: {step_s * 1e3:.3f} ms/step, pass1 {p1_ms:.3f} ms, pass2 {p2_ms:.3f} ms")
+    f64_s, f64_p1, f64_p2, _ = run_device(ctx, vol, R, "fp64", max(3, args.steps // 4), 1)
+    log(f"fp64: {f64_s * 1e3:.3f} ms/step, pass1 {f64_p1:.3f} ms, pass2 {f64_p2:.3f} ms")
+
+    m, cpu = (None, None)
+    if not args.no_cpu:
+        m, cpu = cpu_baseline(vol, R, args.cpu_threads, args.cpu_crop)
+        log(f"cpu baseline: {cpu['value']:.3g} columns/s")
+
+    E = 4
+    s_in = vol.num_segments() / ncols
+    s_out = count / 2 / ncols
+    roofline = None
+    if m is not None:
+        b1 = ncols * ((2 * s_in * E + 4) + (3 * m * E + 4))
+        b2 = ncols * ((3 * m * E + 4) + (2 * s_out * E + 4))
+        name, b, ms = ("pass2", b2, p2_ms) if p2_ms >= p1_ms else ("pass1", b1, p1_ms)
+        achieved = b / (ms * 1e-3) / 1e9
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", f"traffic_{name}_1024_r16.json")
+        if os.path.exists(tpath):
+            traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": name,
+                    "kernel_ms": round(ms, 4), "algorithmic_bytes": int(b),
+                    "step_algorithmic_GBs": round((b1 + b2) / step_s / 1e9, 2)}
+
+    out = {
+        "metric": METRIC,
+        "value": round(ncols / step_s, 1),
+        "unit": "dexel columns/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(step_s * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"direct dexel volume {side}x{side}, z 1024, Poisson(6) segments/column, "
+                               f"Exp(40) gaps and lengths, dilation r={R:g}, one volume per launch",
+                   "grid": [side, side], "radius": R, "storage": "fp32", "op": "dilation",
+                   "pass1_ms": round(p1_ms, 4), "pass2_ms": round(p2_ms, 4),
+                   "plain_fp64": {"value": round(ncols / f64_s, 1), "ms_per_step": round(f64_s * 1e3, 4)},
+                   "s_in": round(s_in, 4), "s_out": round(s_out, 4), "m_oracle": m},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,86 @@
+/* voroffset_hip.h -- C ABI of the MI355X dexel morphology library.
+ *
+ * Replaces, for the `ours` method of geometryprocessing/voroffset, the plugin
+ * interface `vor3d::VoronoiMorpho` (src/vor3d/Voronoi.h:13-44):
+ *   - VoronoiMorpho::dilation(CompressedVolume input, CompressedVolume& result,
+ *       double radius, double& time_1, double& time_2)   Voronoi.h:18
+ *   - VoronoiMorpho::erosion(...)                          Voronoi.h:29, Voronoi.cpp:8-17
+ *   - closing / opening as composed by the CLI            app/cli3d/offset3d.cpp:124-133
+ * and its data layout `vor3d::CompressedVolume` (src/vor3d/CompressedVolume.h:10-41):
+ * a grid nx x ny of z-columns indexed x + nx*y, each a sorted list of endpoints
+ * [b0, e0, b1, e1, ...] in dexel units of absolute z.  Here a volume is a CSR
+ * pair: `offsets` (nx*ny + 1 int64) and `z` (float or double endpoints).
+ *
+ * Conventions (mirroring the reference):
+ *   - radius is in dexels; time_1 / time_2 report pass 1 / pass 2 in ms;
+ *   - erosion complements on a grid grown by one ring of full columns over
+ *     [z_lo - 1, z_hi + 1], dilates, crops and complements within [z_lo, z_hi],
+ *     where z_lo = origin_z / spacing and z_hi = z_lo + 2*padding + extent_z / spacing;
+ *   - output columns are sorted and disjoint, touching intervals fused.
+ * Errors are reported as a nonzero status; vo_last_error() gives the text
+ * (the reference throws std::runtime_error from vor_assert, Common.cpp:8-18).
+ * One context per host thread; calls on a context are serialised on its stream.
+ */
+#ifndef VOROFFSET_HIP_H
+#define VOROFFSET_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vo_context vo_context;
+typedef struct vo_result vo_result;
+
+enum vo_status {
+  VO_OK = 0,
+  VO_ERR_ARGUMENT = 1, /* bad shape, radius, unsorted or odd column, ... */
+  VO_ERR_DEVICE = 2,   /* HIP runtime error */
+  VO_ERR_CAPACITY = 3, /* a sweep exceeded its working-set capacity */
+  VO_ERR_RANGE = 4     /* erosion input endpoint outside [z_lo - 1, z_hi + 1] */
+};
+
+enum vo_op { VO_OP_NOOP = 0, VO_OP_DILATION = 1, VO_OP_EROSION = 2, VO_OP_CLOSING = 3, VO_OP_OPENING = 4 };
+
+/* Storage precision of the endpoints on the device.  Predicates and endpoint
+ * arithmetic are FP64 in both modes; VO_FP32 halves the bytes moved. */
+enum vo_precision { VO_FP32 = 0, VO_FP64 = 1 };
+
+typedef struct {
+  int32_t nx, ny;          /* grid size */
+  const int64_t* offsets;  /* nx*ny + 1 column offsets into z */
+  const void* z;           /* endpoints: float (VO_FP32) or double (VO_FP64) */
+  double z_lo, z_hi;       /* z-range used by erosion (see above) */
+} vo_volume;
+
+/* Context bound to HIP device `device`: owns a stream and a workspace pool. */
+int vo_create(vo_context** ctx, int device);
+void vo_destroy(vo_context* ctx);
+const char* vo_last_error(const vo_context* ctx);
+
+/* Host volume in (z as double), host result out.  With VO_FP32 the endpoints
+ * travel as float relative to an integer z-shift (the rounded centre of
+ * [z_lo, z_hi]) and come back as double.  Replaces VoronoiMorpho::dilation /
+ * erosion and the CLI's closing / opening composition. */
+int vo_morph(vo_context* ctx, int op, int precision, double radius, const vo_volume* in,
+             vo_result** out, double* time_1_ms, double* time_2_ms);
+int vo_result_shape(const vo_result* r, int32_t* nx, int32_t* ny, int64_t* count);
+int vo_result_copy(const vo_result* r, int64_t* offsets, double* z);
+void vo_result_free(vo_result* r);
+
+/* Device volume in (HBM pointers, z of the given precision), device result
+ * out: `out` receives pointers into the context's workspace, valid until the
+ * next call on `ctx`.  `*out_count` is the number of endpoints. */
+int vo_morph_device(vo_context* ctx, int op, int precision, double radius, const vo_volume* in,
+                    vo_volume* out, int64_t* out_count, double* time_1_ms, double* time_2_ms);
+
+/* Statistics of the last dilation on `ctx` (for the bandwidth figures):
+ * number of pass-1 triples written and pass-2 output endpoints. */
+int vo_last_stats(const vo_context* ctx, int64_t* pass1_triples, int64_t* output_endpoints);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VOROFFSET_HIP_H */

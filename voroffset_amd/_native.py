@@ -1,0 +1,132 @@
+"""ctypes binding of libvoroffset_hip.so (the C ABI in include/voroffset_hip.h).
+
+The library is loaded from this package's directory; a missing or unloadable
+library raises immediately -- there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvoroffset_hip.so")
+
+OK, ERR_ARGUMENT, ERR_DEVICE, ERR_CAPACITY, ERR_RANGE = 0, 1, 2, 3, 4
+OP_CODES = {"noop": 0, "dilation": 1, "erosion": 2, "closing": 3, "opening": 4}
+PRECISIONS = {"fp32": 0, "fp64": 1}
+
+# every symbol the header declares
+EXPORTS = ("vo_create", "vo_destroy", "vo_last_error", "vo_morph", "vo_result_shape", "vo_result_copy",
+           "vo_result_free", "vo_morph_device", "vo_last_stats")
+
+
+class VoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[vo status {code}] {msg}")
+        self.code = code
+
+
+class VoVolume(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("offsets", ctypes.c_void_p),
+                ("z", ctypes.c_void_p), ("z_lo", ctypes.c_double), ("z_hi", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"native library missing: {LIB_PATH} (build with python -m voroffset_amd.build)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
+        L.vo_create.argtypes = [ctypes.POINTER(vp), i32]
+        L.vo_create.restype = i32
+        L.vo_destroy.argtypes = [vp]
+        L.vo_destroy.restype = None
+        L.vo_last_error.argtypes = [vp]
+        L.vo_last_error.restype = ctypes.c_char_p
+        L.vo_morph.argtypes = [vp, i32, i32, dbl, ctypes.POINTER(VoVolume), ctypes.POINTER(vp),
+                               ctypes.POINTER(dbl), ctypes.POINTER(dbl)]
+        L.vo_morph.restype = i32
+        L.vo_result_shape.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                      ctypes.POINTER(i64)]
+        L.vo_result_shape.restype = i32
+        L.vo_result_copy.argtypes = [vp, vp, vp]
+        L.vo_result_copy.restype = i32
+        L.vo_result_free.argtypes = [vp]
+        L.vo_result_free.restype = None
+        L.vo_morph_device.argtypes = [vp, i32, i32, dbl, ctypes.POINTER(VoVolume), ctypes.POINTER(VoVolume),
+                                      ctypes.POINTER(i64), ctypes.POINTER(dbl), ctypes.POINTER(dbl)]
+        L.vo_morph_device.restype = i32
+        L.vo_last_stats.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+        L.vo_last_stats.restype = i32
+        _lib = L
+    return _lib
+
+
+class Context:
+    """A device context (stream + workspace pool) of the native library."""
+
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        rc = lib().vo_create(ctypes.byref(self._h), int(device))
+        if rc != OK:
+            raise VoError(rc, "vo_create failed")
+
+    def close(self):
+        if self._h:
+            lib().vo_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _raise(self, rc):
+        raise VoError(rc, lib().vo_last_error(self._h).decode())
+
+    def morph_host(self, op, precision, radius, nx, ny, off, z, z_lo, z_hi):
+        """Host CSR (float64) in -> host CSR (float64) out, plus (time_1, time_2) in ms."""
+        off = np.ascontiguousarray(off, np.int64)
+        z = np.ascontiguousarray(z, np.float64)
+        vol = VoVolume(nx, ny, off.ctypes.data, z.ctypes.data if z.size else None, z_lo, z_hi)
+        res = ctypes.c_void_p()
+        t1, t2 = ctypes.c_double(0), ctypes.c_double(0)
+        rc = lib().vo_morph(self._h, OP_CODES[op], PRECISIONS[precision], float(radius), ctypes.byref(vol),
+                            ctypes.byref(res), ctypes.byref(t1), ctypes.byref(t2))
+        if rc != OK:
+            self._raise(rc)
+        try:
+            rnx, rny, cnt = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
+            lib().vo_result_shape(res, ctypes.byref(rnx), ctypes.byref(rny), ctypes.byref(cnt))
+            o = np.empty(rnx.value * rny.value + 1, np.int64)
+            zz = np.empty(cnt.value, np.float64)
+            lib().vo_result_copy(res, o.ctypes.data, zz.ctypes.data if zz.size else None)
+        finally:
+            lib().vo_result_free(res)
+        return o, zz, (t1.value, t2.value)
+
+    def morph_device(self, op, precision, radius, nx, ny, off_ptr, z_ptr, z_lo, z_hi):
+        """Device pointers in -> (out VoVolume, count, (time_1, time_2)); the
+        output lives in the context's workspace until the next call."""
+        vin = VoVolume(nx, ny, off_ptr, z_ptr, z_lo, z_hi)
+        vout = VoVolume()
+        cnt = ctypes.c_int64()
+        t1, t2 = ctypes.c_double(0), ctypes.c_double(0)
+        rc = lib().vo_morph_device(self._h, OP_CODES[op], PRECISIONS[precision], float(radius), ctypes.byref(vin),
+                                   ctypes.byref(vout), ctypes.byref(cnt), ctypes.byref(t1), ctypes.byref(t2))
+        if rc != OK:
+            self._raise(rc)
+        return vout, cnt.value, (t1.value, t2.value)
+
+    def last_stats(self):
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        lib().vo_last_stats(self._h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value

@@ -1,0 +1,501 @@
+// vo_api.hip -- host side of the C ABI declared in include/voroffset_hip.h.
+//
+// Orchestration of one dilation (VoronoiMorphoVorPower::dilation,
+// src/vor3d/VoronoiVorPower.cpp:24-96) on the device:
+//   pass 1 kernel  -> per-column triples (lo, hi, dy) in a chunked arena
+//   pass 2 kernel  -> per-column output intervals in a chunked arena
+//   exclusive scan of the output lengths + gather -> CSR result
+// Erosion (Voronoi.cpp:8-17) adds a ring-complement kernel before and a
+// crop-complement kernel after; closing / opening chain the two on the
+// device (offset3d.cpp:124-133).  A workspace pool in the context keeps every
+// buffer alive between calls, so repeated calls of the same size allocate
+// nothing.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/voroffset_hip.h"
+#include "vo_sweeps.hpp"
+
+namespace {
+
+// Kernel shapes.  The LDS footprint of each instantiation is checked below.
+constexpr int kP1Cap = 128, kP1NCap = 64;
+constexpr int kP2SCap = 128, kP2PCap = 256, kP2NCap = 64, kP2OCap = 32;
+
+template <class T>
+struct Shapes;
+template <>
+struct Shapes<float> {
+  static constexpr int p1_arena = 2048, p1_cmax = 64;
+  static constexpr int p2_farena = 512, p2_cmax = 64;
+};
+template <>
+struct Shapes<double> {
+  static constexpr int p1_arena = 1024, p1_cmax = 32;
+  static constexpr int p2_farena = 256, p2_cmax = 32;
+};
+
+template <class T>
+using P1S = vo::Pass1Smem<T, kP1Cap, kP1NCap, Shapes<T>::p1_arena, Shapes<T>::p1_cmax>;
+template <class T>
+using P2S = vo::Pass2Smem<T, kP2SCap, kP2PCap, kP2NCap, kP2OCap, Shapes<T>::p2_farena, Shapes<T>::p2_cmax>;
+static_assert(sizeof(P1S<float>) <= 65536, "pass-1 LDS (fp32)");
+static_assert(sizeof(P1S<double>) <= 65536, "pass-1 LDS (fp64)");
+static_assert(sizeof(P2S<float>) <= 65536, "pass-2 LDS (fp32)");
+static_assert(sizeof(P2S<double>) <= 65536, "pass-2 LDS (fp64)");
+
+struct Buffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+enum Slot {
+  kM_LO, kM_HI, kM_DY, kM_START, kM_CNT,  // pass-1 output
+  kO_Z, kO_START, kO_LEN,                  // pass-2 output
+  kSTATUS, kSCAN_TMP,
+  kV0_OFF, kV0_Z, kV1_OFF, kV1_Z, kV2_OFF, kV2_Z, kV3_OFF, kV3_Z,  // device volumes
+  kIN_OFF, kIN_Z,                                                   // staged host input
+  kSLOTS
+};
+
+struct DevVol {
+  int nx = 0, ny = 0;
+  const int64_t* off = nullptr;
+  const void* z = nullptr;
+  int64_t count = 0;
+};
+
+}  // namespace
+
+struct vo_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  Buffer buf[kSLOTS];
+  int64_t* host_words = nullptr;  // pinned: status + counts
+  std::string error;
+  int64_t last_triples = 0, last_outputs = 0;
+  int p1_cap_per_col = 64;  // pass-1 arena triples per column (grows on overflow)
+  int p2_cap_per_col = 16;  // pass-2 arena endpoints per column (grows on overflow)
+};
+
+struct vo_result {
+  int32_t nx = 0, ny = 0;
+  std::vector<int64_t> off;
+  std::vector<double> z;
+};
+
+namespace {
+
+struct Failure {
+  int code;
+};
+
+void fail(vo_context* ctx, int code, const std::string& msg) {
+  ctx->error = msg;
+  throw Failure{code};
+}
+
+void check(vo_context* ctx, hipError_t e, const char* what) {
+  if (e != hipSuccess) fail(ctx, VO_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void* ws(vo_context* ctx, int slot, size_t bytes) {
+  Buffer& b = ctx->buf[slot];
+  if (bytes == 0) bytes = 16;
+  if (b.bytes < bytes) {
+    if (b.ptr) {
+      check(ctx, hipStreamSynchronize(ctx->stream), "sync");  // the old buffer may still be in use
+      check(ctx, hipFree(b.ptr), "hipFree");
+    }
+    b.ptr = nullptr;
+    const size_t want = bytes + bytes / 4;
+    check(ctx, hipMalloc(&b.ptr, want), "hipMalloc");
+    b.bytes = want;
+  }
+  return b.ptr;
+}
+
+unsigned blocks_for(int64_t n, int threads) { return unsigned((n + threads - 1) / threads); }
+
+// Exclusive scan of len[0..n] (len[n] must be 0) into off[0..n].
+void exclusive_scan(vo_context* ctx, const int64_t* len, int64_t* off, int64_t n) {
+  size_t tmp = 0;
+  check(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, len, off, int(n), ctx->stream), "scan size");
+  void* t = ws(ctx, kSCAN_TMP, tmp);
+  check(ctx, hipcub::DeviceScan::ExclusiveSum(t, tmp, len, off, int(n), ctx->stream), "scan");
+}
+
+int64_t read_word(vo_context* ctx, const int64_t* dptr) {
+  check(ctx, hipMemcpyAsync(ctx->host_words, dptr, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream),
+        "copy word");
+  check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+  return ctx->host_words[0];
+}
+
+const char* status_text(int s) {
+  if (s & vo::kStatusListOverflow) return "a sweep list exceeded its LDS capacity";
+  if (s & vo::kStatusColumnOverflow) return "a column exceeded the staging capacity";
+  return "arena overflow";
+}
+
+// One dilation: `in` -> CSR in (out_off_slot, out_z_slot).
+template <class T>
+DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int out_z_slot, double* t1,
+              double* t2) {
+  const int nx = in.nx, ny = in.ny;
+  const int64_t ncols = int64_t(nx) * ny;
+  if (!(R >= 0) || !std::isfinite(R)) fail(ctx, VO_ERR_ARGUMENT, "radius must be finite and >= 0");
+  if (R > 60000) fail(ctx, VO_ERR_ARGUMENT, "radius too large");
+  const int rint = int(std::floor(R));
+  const double R2 = R * R;
+  int* status = static_cast<int*>(ws(ctx, kSTATUS, 64));
+  int64_t* m_start = static_cast<int64_t*>(ws(ctx, kM_START, ncols * sizeof(int64_t)));
+  int32_t* m_cnt = static_cast<int32_t*>(ws(ctx, kM_CNT, ncols * sizeof(int32_t)));
+  int64_t* o_start = static_cast<int64_t*>(ws(ctx, kO_START, ncols * sizeof(int64_t)));
+  int64_t* o_len = static_cast<int64_t*>(ws(ctx, kO_LEN, (ncols + 1) * sizeof(int64_t)));
+  int64_t* out_off = static_cast<int64_t*>(ws(ctx, out_off_slot, (ncols + 1) * sizeof(int64_t)));
+  DevVol out;
+  out.nx = nx;
+  out.ny = ny;
+  out.off = out_off;
+  if (ncols == 0) {
+    check(ctx, hipMemsetAsync(out_off, 0, sizeof(int64_t), ctx->stream), "memset");
+    out.z = ws(ctx, out_z_slot, 16);
+    return out;
+  }
+  int chunk1 = std::min(Shapes<T>::p1_cmax, 32);
+  int chunk2 = Shapes<T>::p2_cmax;
+  for (int attempt = 0;; ++attempt) {
+    const int nch1 = (ny + chunk1 - 1) / chunk1;
+    const int64_t arena1 = int64_t(chunk1) * ctx->p1_cap_per_col;
+    const int64_t grid1 = int64_t(nx) * nch1;
+    T* m_lo = static_cast<T*>(ws(ctx, kM_LO, grid1 * arena1 * sizeof(T)));
+    T* m_hi = static_cast<T*>(ws(ctx, kM_HI, grid1 * arena1 * sizeof(T)));
+    uint16_t* m_dy = static_cast<uint16_t*>(ws(ctx, kM_DY, grid1 * arena1 * sizeof(uint16_t)));
+    const int nch2 = (nx + chunk2 - 1) / chunk2;
+    const int64_t arena2 = int64_t(chunk2) * ctx->p2_cap_per_col;
+    const int64_t grid2 = int64_t(ny) * nch2;
+    T* o_z = static_cast<T*>(ws(ctx, kO_Z, grid2 * arena2 * sizeof(T)));
+    check(ctx, hipMemsetAsync(status, 0, sizeof(int), ctx->stream), "memset status");
+    if (grid1 > INT32_MAX || grid2 > INT32_MAX) fail(ctx, VO_ERR_ARGUMENT, "grid too large");
+
+    vo::Pass1Args<T> a1{nx, ny, rint, chunk1, nch1, in.off, static_cast<const T*>(in.z), m_lo, m_hi, m_dy,
+                        arena1, m_start, m_cnt, status};
+    check(ctx, hipEventRecord(ctx->ev[0], ctx->stream), "event");
+    hipLaunchKernelGGL((vo::pass1_kernel<T, kP1Cap, kP1NCap, Shapes<T>::p1_arena, Shapes<T>::p1_cmax>),
+                       dim3(unsigned(grid1)), dim3(vo::kWave), 0, ctx->stream, a1);
+    check(ctx, hipGetLastError(), "pass1 launch");
+    check(ctx, hipEventRecord(ctx->ev[1], ctx->stream), "event");
+    vo::Pass2Args<T> a2{nx, ny, rint, R2, chunk2, nch2, m_lo, m_hi, m_dy, m_start, m_cnt, grid1 * arena1,
+                        o_z, arena2, o_start, o_len, status};
+    hipLaunchKernelGGL((vo::pass2_kernel<T, kP2SCap, kP2PCap, kP2NCap, kP2OCap, Shapes<T>::p2_farena,
+                                         Shapes<T>::p2_cmax>),
+                       dim3(unsigned(grid2)), dim3(vo::kWave), 0, ctx->stream, a2);
+    check(ctx, hipGetLastError(), "pass2 launch");
+    check(ctx, hipEventRecord(ctx->ev[2], ctx->stream), "event");
+    check(ctx, hipMemsetAsync(o_len + ncols, 0, sizeof(int64_t), ctx->stream), "memset");
+    check(ctx, hipMemcpyAsync(ctx->host_words + 1, status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
+          "copy status");
+    check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    const int st = int(ctx->host_words[1] & 0xffffffff);
+    if (st == 0) {
+      if (t1 || t2) {
+        float ms1 = 0, ms2 = 0;
+        check(ctx, hipEventElapsedTime(&ms1, ctx->ev[0], ctx->ev[1]), "elapsed");
+        check(ctx, hipEventElapsedTime(&ms2, ctx->ev[1], ctx->ev[2]), "elapsed");
+        if (t1) *t1 = ms1;
+        if (t2) *t2 = ms2;
+      }
+      break;
+    }
+    if (st & (vo::kStatusListOverflow | vo::kStatusColumnOverflow))
+      fail(ctx, VO_ERR_CAPACITY, status_text(st));
+    // arena overflow: give each column more room (LDS arenas: shorter chunks)
+    if (attempt >= 6) fail(ctx, VO_ERR_CAPACITY, "arena overflow after retries");
+    if (chunk1 > 8) chunk1 /= 2;
+    if (chunk2 > 8) chunk2 /= 2;
+    ctx->p1_cap_per_col *= 2;
+    ctx->p2_cap_per_col *= 2;
+  }
+  // output CSR
+  exclusive_scan(ctx, o_len, out_off, ncols + 1);
+  const int64_t total = read_word(ctx, out_off + ncols);
+  T* oz = static_cast<T*>(ws(ctx, out_z_slot, size_t(std::max<int64_t>(total, 1)) * sizeof(T)));
+  hipLaunchKernelGGL(vo::gather_kernel<T>, dim3(blocks_for(ncols, 256)), dim3(256), 0, ctx->stream, ncols,
+                     static_cast<const T*>(ctx->buf[kO_Z].ptr), o_start, o_len, out_off, oz);
+  check(ctx, hipGetLastError(), "gather launch");
+  out.z = oz;
+  out.count = total;
+  ctx->last_outputs = total;
+  return out;
+}
+
+// Per-column complement kernels (phase 0: lengths, phase 1: data): the ring
+// complement grows the grid by one column on each side, the crop shrinks it.
+template <class T>
+DevVol column_transform(vo_context* ctx, bool ring, const DevVol& in, T lo, T hi, int off_slot, int z_slot) {
+  const int onx = ring ? in.nx + 2 : in.nx - 2, ony = ring ? in.ny + 2 : in.ny - 2;
+  const int64_t n = int64_t(onx) * ony;
+  int64_t* len = static_cast<int64_t*>(ws(ctx, kO_LEN, (n + 1) * sizeof(int64_t)));
+  int64_t* off = static_cast<int64_t*>(ws(ctx, off_slot, (n + 1) * sizeof(int64_t)));
+  auto launch = [&](int phase, const int64_t* o, T* z) {
+    const dim3 grid(blocks_for(std::max<int64_t>(n, 1), 256));
+    if (ring)
+      hipLaunchKernelGGL(vo::ring_complement_kernel<T>, grid, dim3(256), 0, ctx->stream, in.nx, in.ny, in.off,
+                         static_cast<const T*>(in.z), lo, hi, phase, len, o, z);
+    else
+      hipLaunchKernelGGL(vo::crop_complement_kernel<T>, grid, dim3(256), 0, ctx->stream, onx, ony, in.off,
+                         static_cast<const T*>(in.z), lo, hi, phase, len, o, z);
+    check(ctx, hipGetLastError(), "complement launch");
+  };
+  launch(0, nullptr, nullptr);
+  check(ctx, hipMemsetAsync(len + n, 0, sizeof(int64_t), ctx->stream), "memset");
+  exclusive_scan(ctx, len, off, n + 1);
+  const int64_t total = read_word(ctx, off + n);
+  T* z = static_cast<T*>(ws(ctx, z_slot, size_t(std::max<int64_t>(total, 1)) * sizeof(T)));
+  launch(1, off, z);
+  DevVol out;
+  out.nx = onx;
+  out.ny = ony;
+  out.off = off;
+  out.z = z;
+  out.count = total;
+  return out;
+}
+
+// erosion = crop-complement o dilation o ring-complement (Voronoi.cpp:8-17).
+template <class T>
+DevVol erode(vo_context* ctx, const DevVol& in, double R, double zlo, double zhi, int work_a, int work_b,
+             int out_slot, double* t1, double* t2) {
+  const double lo = zlo - 1, hi = zhi + 1;
+  DevVol g = column_transform<T>(ctx, true, in, T(lo), T(hi), work_a, work_a + 1);
+  DevVol d = dilate<T>(ctx, g, R, work_b, work_b + 1, t1, t2);
+  return column_transform<T>(ctx, false, d, T(lo + 1), T(hi - 1), out_slot, out_slot + 1);
+}
+
+template <class T>
+DevVol run_op(vo_context* ctx, int op, const DevVol& in, double R, double zlo, double zhi, double* t1,
+              double* t2) {
+  switch (op) {
+    case VO_OP_DILATION:
+      return dilate<T>(ctx, in, R, kV3_OFF, kV3_Z, t1, t2);
+    case VO_OP_EROSION:
+      return erode<T>(ctx, in, R, zlo, zhi, kV0_OFF, kV1_OFF, kV3_OFF, t1, t2);
+    case VO_OP_CLOSING: {
+      DevVol d = dilate<T>(ctx, in, R, kV2_OFF, kV2_Z, t1, t2);
+      return erode<T>(ctx, d, R, zlo, zhi, kV0_OFF, kV1_OFF, kV3_OFF, t1, t2);
+    }
+    case VO_OP_OPENING: {
+      DevVol e = erode<T>(ctx, in, R, zlo, zhi, kV0_OFF, kV1_OFF, kV2_OFF, t1, t2);
+      return dilate<T>(ctx, e, R, kV3_OFF, kV3_Z, t1, t2);
+    }
+    default:
+      fail(ctx, VO_ERR_ARGUMENT, "unknown operation");
+  }
+  return in;
+}
+
+int guarded(vo_context* ctx, const std::function<void()>& body) {
+  if (!ctx) return VO_ERR_ARGUMENT;
+  try {
+    check(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    body();
+    return VO_OK;
+  } catch (const Failure& f) {
+    return f.code;
+  } catch (const std::exception& e) {
+    ctx->error = e.what();
+    return VO_ERR_ARGUMENT;
+  }
+}
+
+void validate_shape(vo_context* ctx, const vo_volume* v) {
+  if (!v || v->nx < 0 || v->ny < 0 || (!v->offsets && int64_t(v->nx) * v->ny > 0))
+    fail(ctx, VO_ERR_ARGUMENT, "invalid volume");
+}
+
+// Host-side checks of the CompressedVolume invariants the sweeps rely on.
+void validate_host(vo_context* ctx, const vo_volume* v, int op, double radius) {
+  const int64_t n = int64_t(v->nx) * v->ny;
+  const double* z = static_cast<const double*>(v->z);
+  if (v->offsets[0] != 0) fail(ctx, VO_ERR_ARGUMENT, "offsets[0] must be 0");
+  for (int64_t c = 0; c < n; ++c) {
+    const int64_t s = v->offsets[c], e = v->offsets[c + 1];
+    if (e < s || (e - s) % 2) fail(ctx, VO_ERR_ARGUMENT, "column " + std::to_string(c) + " has odd length");
+    for (int64_t i = s; i < e; ++i) {
+      if (!std::isfinite(z[i])) fail(ctx, VO_ERR_ARGUMENT, "non-finite endpoint");
+      if (i > s && z[i] < z[i - 1]) fail(ctx, VO_ERR_ARGUMENT, "column " + std::to_string(c) + " is unsorted");
+      if (op == VO_OP_EROSION || op == VO_OP_OPENING) {
+        if (z[i] < v->z_lo - 1 || z[i] > v->z_hi + 1)
+          fail(ctx, VO_ERR_RANGE, "erosion input endpoint outside [z_lo - 1, z_hi + 1]");
+      }
+      if (op == VO_OP_CLOSING && (z[i] - radius < v->z_lo - 1 || z[i] + radius > v->z_hi + 1))
+        fail(ctx, VO_ERR_RANGE, "closing: the dilated volume would leave [z_lo - 1, z_hi + 1]");
+    }
+  }
+}
+
+template <class T>
+void morph_host(vo_context* ctx, int op, double R, const vo_volume* in, vo_result** out, double* t1,
+                double* t2) {
+  validate_shape(ctx, in);
+  validate_host(ctx, in, op, R);
+  const int64_t n = int64_t(in->nx) * in->ny;
+  const int64_t cnt = n ? in->offsets[n] : 0;
+  // FP32 travels relative to an integer shift of the z-range centre
+  const double shift = std::is_same<T, float>::value ? std::round(0.5 * (in->z_lo + in->z_hi)) : 0.0;
+  std::vector<T> zt(size_t(std::max<int64_t>(cnt, 1)));
+  const double* z = static_cast<const double*>(in->z);
+  for (int64_t i = 0; i < cnt; ++i) zt[i] = T(z[i] - shift);
+  int64_t* doff = static_cast<int64_t*>(ws(ctx, kIN_OFF, (n + 1) * sizeof(int64_t)));
+  T* dz = static_cast<T*>(ws(ctx, kIN_Z, zt.size() * sizeof(T)));
+  check(ctx, hipMemcpyAsync(doff, in->offsets, (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream),
+        "upload");
+  check(ctx, hipMemcpyAsync(dz, zt.data(), zt.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
+  DevVol v;
+  v.nx = in->nx;
+  v.ny = in->ny;
+  v.off = doff;
+  v.z = dz;
+  v.count = cnt;
+  DevVol r = v;
+  if (op != VO_OP_NOOP) r = run_op<T>(ctx, op, v, R, in->z_lo - shift, in->z_hi - shift, t1, t2);
+  auto* res = new vo_result;
+  res->nx = r.nx;
+  res->ny = r.ny;
+  const int64_t rn = int64_t(r.nx) * r.ny;
+  res->off.resize(rn + 1);
+  std::vector<T> rz(size_t(std::max<int64_t>(r.count, 1)));
+  check(ctx, hipMemcpyAsync(res->off.data(), r.off, (rn + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream),
+        "download");
+  check(ctx, hipMemcpyAsync(rz.data(), r.z, rz.size() * sizeof(T), hipMemcpyDeviceToHost, ctx->stream), "download");
+  check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+  res->z.resize(r.count);
+  for (int64_t i = 0; i < r.count; ++i) res->z[i] = double(rz[i]) + shift;
+  *out = res;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vo_create(vo_context** out, int device) {
+  if (!out) return VO_ERR_ARGUMENT;
+  *out = nullptr;
+  auto* ctx = new vo_context;
+  ctx->device = device;
+  int rc = guarded(ctx, [&] {
+    int n = 0;
+    check(ctx, hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (device < 0 || device >= n) fail(ctx, VO_ERR_ARGUMENT, "no such HIP device");
+    check(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "stream");
+    for (auto& e : ctx->ev) check(ctx, hipEventCreate(&e), "event");
+    check(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->host_words), 4 * sizeof(int64_t), hipHostMallocDefault),
+          "pinned");
+  });
+  if (rc != VO_OK) {
+    static thread_local std::string last;
+    last = ctx->error;
+    vo_destroy(ctx);
+    return rc;
+  }
+  *out = ctx;
+  return VO_OK;
+}
+
+void vo_destroy(vo_context* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto& b : ctx->buf)
+    if (b.ptr) (void)hipFree(b.ptr);
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->host_words) (void)hipHostFree(ctx->host_words);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* vo_last_error(const vo_context* ctx) { return ctx ? ctx->error.c_str() : "no context"; }
+
+int vo_morph(vo_context* ctx, int op, int precision, double radius, const vo_volume* in, vo_result** out,
+             double* time_1_ms, double* time_2_ms) {
+  return guarded(ctx, [&] {
+    if (!out) fail(ctx, VO_ERR_ARGUMENT, "null result pointer");
+    if (op < VO_OP_NOOP || op > VO_OP_OPENING) fail(ctx, VO_ERR_ARGUMENT, "unknown operation");
+    if (precision == VO_FP32)
+      morph_host<float>(ctx, op, radius, in, out, time_1_ms, time_2_ms);
+    else if (precision == VO_FP64)
+      morph_host<double>(ctx, op, radius, in, out, time_1_ms, time_2_ms);
+    else
+      fail(ctx, VO_ERR_ARGUMENT, "unknown precision");
+  });
+}
+
+int vo_result_shape(const vo_result* r, int32_t* nx, int32_t* ny, int64_t* count) {
+  if (!r) return VO_ERR_ARGUMENT;
+  if (nx) *nx = r->nx;
+  if (ny) *ny = r->ny;
+  if (count) *count = int64_t(r->z.size());
+  return VO_OK;
+}
+
+int vo_result_copy(const vo_result* r, int64_t* offsets, double* z) {
+  if (!r) return VO_ERR_ARGUMENT;
+  if (offsets) std::memcpy(offsets, r->off.data(), r->off.size() * sizeof(int64_t));
+  if (z && !r->z.empty()) std::memcpy(z, r->z.data(), r->z.size() * sizeof(double));
+  return VO_OK;
+}
+
+void vo_result_free(vo_result* r) { delete r; }
+
+int vo_morph_device(vo_context* ctx, int op, int precision, double radius, const vo_volume* in, vo_volume* out,
+                    int64_t* out_count, double* time_1_ms, double* time_2_ms) {
+  return guarded(ctx, [&] {
+    validate_shape(ctx, in);
+    if (!out || !out_count) fail(ctx, VO_ERR_ARGUMENT, "null output");
+    DevVol v;
+    v.nx = in->nx;
+    v.ny = in->ny;
+    v.off = in->offsets;
+    v.z = in->z;
+    DevVol r = v;
+    if (op == VO_OP_NOOP) {
+      const int64_t n = int64_t(in->nx) * in->ny;
+      r.count = n ? read_word(ctx, in->offsets + n) : 0;
+    } else if (precision == VO_FP32) {
+      r = run_op<float>(ctx, op, v, radius, in->z_lo, in->z_hi, time_1_ms, time_2_ms);
+    } else if (precision == VO_FP64) {
+      r = run_op<double>(ctx, op, v, radius, in->z_lo, in->z_hi, time_1_ms, time_2_ms);
+    } else {
+      fail(ctx, VO_ERR_ARGUMENT, "unknown precision");
+    }
+    check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    out->nx = r.nx;
+    out->ny = r.ny;
+    out->offsets = r.off;
+    out->z = r.z;
+    out->z_lo = in->z_lo;
+    out->z_hi = in->z_hi;
+    *out_count = r.count;
+  });
+}
+
+int vo_last_stats(const vo_context* ctx, int64_t* pass1_triples, int64_t* output_endpoints) {
+  if (!ctx) return VO_ERR_ARGUMENT;
+  if (pass1_triples) *pass1_triples = ctx->last_triples;
+  if (output_endpoints) *output_endpoints = ctx->last_outputs;
+  return VO_OK;
+}
+
+}  // extern "C"
