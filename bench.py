@@ -59,26 +59,27 @@ def cpu_baseline(vol, radius, threads, crop):
 
 
 def run_device(ctx, vol, radius, precision, steps, warmup):
-    import torch
+    from voroffset_amd import _native
 
     zlo, zhi = vol.z_range()
     shift = float(np.round(0.5 * (zlo + zhi))) if precision == "fp32" else 0.0
     dtype = np.float32 if precision == "fp32" else np.float64
-    d_off = torch.from_numpy(vol.off).cuda()
-    d_z = torch.from_numpy((vol.z - shift).astype(dtype)).cuda()
-    torch.cuda.synchronize()
-    args = ("dilation", precision, radius, vol.nx, vol.ny, d_off.data_ptr(), d_z.data_ptr(), zlo - shift, zhi - shift)
+    d_off = _native.DeviceArray(vol.off)
+    d_z = _native.DeviceArray((vol.z - shift).astype(dtype))
+    args = ("dilation", precision, radius, vol.nx, vol.ny, d_off.address, d_z.address, zlo - shift, zhi - shift)
     for _ in range(warmup):
         ctx.morph_device(*args)
-    torch.cuda.synchronize()
+    _native.synchronize()
     p1 = p2 = 0.0
     t0 = time.perf_counter()
     for _ in range(steps):
         _, count, (t1, t2) = ctx.morph_device(*args)
         p1 += t1
         p2 += t2
-    torch.cuda.synchronize()
+    _native.synchronize()
     dt = time.perf_counter() - t0
+    d_off.free()
+    d_z.free()
     return dt / steps, p1 / steps, p2 / steps, count
 
 

@@ -130,3 +130,81 @@ class Context:
         a, b = ctypes.c_int64(), ctypes.c_int64()
         lib().vo_last_stats(self._h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+
+# -- device buffers -------------------------------------------------------
+# Plain hipMalloc'd buffers for callers that keep volumes resident in HBM.
+# They go through the HIP runtime this library is linked against (resolved by
+# soname to the copy already loaded with the library), so one process never
+# mixes two HIP runtimes.
+
+_hip = None
+
+
+def hip():
+    global _hip
+    if _hip is None:
+        lib()
+        H = ctypes.CDLL("libamdhip64.so.7")
+        H.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        H.hipMalloc.restype = ctypes.c_int
+        H.hipFree.argtypes = [ctypes.c_void_p]
+        H.hipFree.restype = ctypes.c_int
+        H.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        H.hipMemcpy.restype = ctypes.c_int
+        H.hipDeviceSynchronize.restype = ctypes.c_int
+        H.hipSetDevice.argtypes = [ctypes.c_int]
+        H.hipSetDevice.restype = ctypes.c_int
+        H.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        H.hipGetDeviceCount.restype = ctypes.c_int
+        _hip = H
+    return _hip
+
+
+def _hip_check(rc, what):
+    if rc != 0:
+        raise VoError(ERR_DEVICE, f"{what} failed with hipError {rc}")
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    rc = hip().hipGetDeviceCount(ctypes.byref(n))
+    return n.value if rc == 0 else 0
+
+
+def synchronize():
+    _hip_check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+class DeviceArray:
+    """A device copy of a host numpy array (freed with free() or on GC)."""
+
+    def __init__(self, host):
+        host = np.ascontiguousarray(host)
+        self.dtype, self.size = host.dtype, host.size
+        self.ptr = ctypes.c_void_p()
+        _hip_check(hip().hipMalloc(ctypes.byref(self.ptr), max(host.nbytes, 16)), "hipMalloc")
+        if host.nbytes:
+            _hip_check(hip().hipMemcpy(self.ptr, host.ctypes.data, host.nbytes, 1), "hipMemcpy H2D")
+
+    @property
+    def address(self):
+        return self.ptr.value
+
+    def free(self):
+        if self.ptr:
+            hip().hipFree(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def to_host(address, dtype, count):
+    out = np.empty(count, dtype)
+    if count:
+        _hip_check(hip().hipMemcpy(out.ctypes.data, ctypes.c_void_p(address), out.nbytes, 2), "hipMemcpy D2H")
+    return out

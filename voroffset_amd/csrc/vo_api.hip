@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <string>
@@ -29,7 +30,9 @@ namespace {
 
 // Kernel shapes.  The LDS footprint of each instantiation is checked below.
 constexpr int kP1Cap = 128, kP1NCap = 64;
-constexpr int kP2SCap = 128, kP2PCap = 256, kP2NCap = 64, kP2OCap = 32;
+constexpr int kP2SCap = 128, kP2PCap = 256, kP2NCap = 64, kP2OCap = 256;
+constexpr int kGOCap = 16;    // gather path: intervals per output column list
+constexpr int kGWMax = 64 + 2 * 128;  // gather path: widest source window (R <= 128)
 
 template <class T>
 struct Shapes;
@@ -37,11 +40,13 @@ template <>
 struct Shapes<float> {
   static constexpr int p1_arena = 2048, p1_cmax = 64;
   static constexpr int p2_farena = 512, p2_cmax = 64;
+  static constexpr int g_pool = 3072;
 };
 template <>
 struct Shapes<double> {
   static constexpr int p1_arena = 1024, p1_cmax = 32;
   static constexpr int p2_farena = 256, p2_cmax = 32;
+  static constexpr int g_pool = 1536;
 };
 
 template <class T>
@@ -52,6 +57,10 @@ static_assert(sizeof(P1S<float>) <= 65536, "pass-1 LDS (fp32)");
 static_assert(sizeof(P1S<double>) <= 65536, "pass-1 LDS (fp64)");
 static_assert(sizeof(P2S<float>) <= 65536, "pass-2 LDS (fp32)");
 static_assert(sizeof(P2S<double>) <= 65536, "pass-2 LDS (fp64)");
+template <class T>
+using P2G = vo::Pass2GSmem<T, Shapes<T>::g_pool, kGOCap, kGWMax>;
+static_assert(sizeof(P2G<float>) <= 65536, "pass-2 gather LDS (fp32)");
+static_assert(sizeof(P2G<double>) <= 65536, "pass-2 gather LDS (fp64)");
 
 struct Buffer {
   void* ptr = nullptr;
@@ -61,7 +70,7 @@ struct Buffer {
 enum Slot {
   kM_LO, kM_HI, kM_DY, kM_START, kM_CNT,  // pass-1 output
   kO_Z, kO_START, kO_LEN,                  // pass-2 output
-  kSTATUS, kSCAN_TMP,
+  kSTATUS, kSCAN_TMP, kTILES,
   kV0_OFF, kV0_Z, kV1_OFF, kV1_Z, kV2_OFF, kV2_Z, kV3_OFF, kV3_Z,  // device volumes
   kIN_OFF, kIN_Z,                                                   // staged host input
   kSLOTS
@@ -86,6 +95,8 @@ struct vo_context {
   int64_t last_triples = 0, last_outputs = 0;
   int p1_cap_per_col = 64;  // pass-1 arena triples per column (grows on overflow)
   int p2_cap_per_col = 16;  // pass-2 arena endpoints per column (grows on overflow)
+  bool force_sweep = false;  // VO_PASS2=sweep: always use the sweep form of pass 2
+  int last_fallback_tiles = 0;
 };
 
 struct vo_result {
@@ -135,6 +146,15 @@ void exclusive_scan(vo_context* ctx, const int64_t* len, int64_t* off, int64_t n
   check(ctx, hipcub::DeviceScan::ExclusiveSum(t, tmp, len, off, int(n), ctx->stream), "scan");
 }
 
+int32_t read_word32(vo_context* ctx, const int* dptr) {
+  check(ctx, hipMemcpyAsync(ctx->host_words + 2, dptr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
+        "copy word");
+  check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+  int32_t v;
+  std::memcpy(&v, ctx->host_words + 2, sizeof(v));
+  return v;
+}
+
 int64_t read_word(vo_context* ctx, const int64_t* dptr) {
   check(ctx, hipMemcpyAsync(ctx->host_words, dptr, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream),
         "copy word");
@@ -142,10 +162,14 @@ int64_t read_word(vo_context* ctx, const int64_t* dptr) {
   return ctx->host_words[0];
 }
 
-const char* status_text(int s) {
-  if (s & vo::kStatusListOverflow) return "a sweep list exceeded its LDS capacity";
-  if (s & vo::kStatusColumnOverflow) return "a column exceeded the staging capacity";
-  return "arena overflow";
+std::string status_text(int s) {
+  std::string t;
+  if (s & vo::kStatusP1List) t += " pass-1 piece list;";
+  if (s & vo::kStatusP1Union) t += " pass-1 forward/backward union;";
+  if (s & vo::kStatusP2Interiors) t += " pass-2 interior list;";
+  if (s & vo::kStatusP2Disks) t += " pass-2 disk list;";
+  if (s & vo::kStatusP2Slice) t += " pass-2 line slice;";
+  return "a sweep exceeded its LDS capacity:" + t;
 }
 
 // One dilation: `in` -> CSR in (out_off_slot, out_z_slot).
@@ -182,12 +206,17 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
     T* m_lo = static_cast<T*>(ws(ctx, kM_LO, grid1 * arena1 * sizeof(T)));
     T* m_hi = static_cast<T*>(ws(ctx, kM_HI, grid1 * arena1 * sizeof(T)));
     uint16_t* m_dy = static_cast<uint16_t*>(ws(ctx, kM_DY, grid1 * arena1 * sizeof(uint16_t)));
-    const int nch2 = (nx + chunk2 - 1) / chunk2;
-    const int64_t arena2 = int64_t(chunk2) * ctx->p2_cap_per_col;
-    const int64_t grid2 = int64_t(ny) * nch2;
-    T* o_z = static_cast<T*>(ws(ctx, kO_Z, grid2 * arena2 * sizeof(T)));
+    const int ntx = (nx + vo::kGatherTile - 1) / vo::kGatherTile;
+    const int64_t ntiles = int64_t(ny) * ntx;
+    const int64_t garena = int64_t(vo::kGatherTile) * 2 * kGOCap;  // endpoints per gather tile
+    const int64_t arena2 = int64_t(chunk2) * ctx->p2_cap_per_col;   // endpoints per sweep chunk
+    const int64_t fb_chunks = ntiles * (vo::kGatherTile / chunk2);  // worst case: every tile falls back
+    const int64_t o_total = ntiles * garena + fb_chunks * arena2;
+    T* o_z = static_cast<T*>(ws(ctx, kO_Z, o_total * sizeof(T)));
+    int* tiles = static_cast<int*>(ws(ctx, kTILES, (ntiles + 1) * sizeof(int)));
     check(ctx, hipMemsetAsync(status, 0, sizeof(int), ctx->stream), "memset status");
-    if (grid1 > INT32_MAX || grid2 > INT32_MAX) fail(ctx, VO_ERR_ARGUMENT, "grid too large");
+    check(ctx, hipMemsetAsync(tiles, 0, sizeof(int), ctx->stream), "memset tiles");
+    if (grid1 > INT32_MAX || fb_chunks > INT32_MAX) fail(ctx, VO_ERR_ARGUMENT, "grid too large");
 
     vo::Pass1Args<T> a1{nx, ny, rint, chunk1, nch1, in.off, static_cast<const T*>(in.z), m_lo, m_hi, m_dy,
                         arena1, m_start, m_cnt, status};
@@ -196,12 +225,27 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
                        dim3(unsigned(grid1)), dim3(vo::kWave), 0, ctx->stream, a1);
     check(ctx, hipGetLastError(), "pass1 launch");
     check(ctx, hipEventRecord(ctx->ev[1], ctx->stream), "event");
-    vo::Pass2Args<T> a2{nx, ny, rint, R2, chunk2, nch2, m_lo, m_hi, m_dy, m_start, m_cnt, grid1 * arena1,
-                        o_z, arena2, o_start, o_len, status};
-    hipLaunchKernelGGL((vo::pass2_kernel<T, kP2SCap, kP2PCap, kP2NCap, kP2OCap, Shapes<T>::p2_farena,
-                                         Shapes<T>::p2_cmax>),
-                       dim3(unsigned(grid2)), dim3(vo::kWave), 0, ctx->stream, a2);
-    check(ctx, hipGetLastError(), "pass2 launch");
+    const bool gather = rint <= 128 && !ctx->force_sweep;
+    int nfall = 0;
+    if (gather) {
+      vo::Pass2GArgs<T> ag{nx, ny, rint, R2, ntx, m_lo, m_hi, m_dy, m_start, m_cnt, grid1 * arena1,
+                           o_z, garena, o_start, o_len, tiles, int(ntiles), status};
+      hipLaunchKernelGGL((vo::pass2_gather_kernel<T, Shapes<T>::g_pool, kGOCap, kGWMax>), dim3(unsigned(ntiles)),
+                         dim3(vo::kWave), 0, ctx->stream, ag);
+      check(ctx, hipGetLastError(), "pass2 gather launch");
+      nfall = int(read_word32(ctx, tiles));
+    }
+    // the sweep form: the whole volume, or the tiles the gather form gave back
+    const int64_t grid2 = gather ? int64_t(nfall) * (vo::kGatherTile / chunk2) : int64_t(ny) * ((nx + chunk2 - 1) / chunk2);
+    if (grid2 > 0) {
+      vo::Pass2Args<T> a2{nx, ny, rint, R2, chunk2, 0, m_lo, m_hi, m_dy, m_start, m_cnt, grid1 * arena1,
+                          o_z, arena2, o_start, o_len, status, gather ? tiles : nullptr, ntiles * garena};
+      hipLaunchKernelGGL((vo::pass2_kernel<T, kP2SCap, kP2PCap, kP2NCap, kP2OCap, Shapes<T>::p2_farena,
+                                           Shapes<T>::p2_cmax>),
+                         dim3(unsigned(grid2)), dim3(vo::kWave), 0, ctx->stream, a2);
+      check(ctx, hipGetLastError(), "pass2 launch");
+    }
+    ctx->last_fallback_tiles = nfall;
     check(ctx, hipEventRecord(ctx->ev[2], ctx->stream), "event");
     check(ctx, hipMemsetAsync(o_len + ncols, 0, sizeof(int64_t), ctx->stream), "memset");
     check(ctx, hipMemcpyAsync(ctx->host_words + 1, status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
@@ -218,8 +262,7 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
       }
       break;
     }
-    if (st & (vo::kStatusListOverflow | vo::kStatusColumnOverflow))
-      fail(ctx, VO_ERR_CAPACITY, status_text(st));
+    if (st & vo::kStatusListOverflow) fail(ctx, VO_ERR_CAPACITY, status_text(st));
     // arena overflow: give each column more room (LDS arenas: shorter chunks)
     if (attempt >= 6) fail(ctx, VO_ERR_CAPACITY, "arena overflow after retries");
     if (chunk1 > 8) chunk1 /= 2;
@@ -394,6 +437,7 @@ int vo_create(vo_context** out, int device) {
   *out = nullptr;
   auto* ctx = new vo_context;
   ctx->device = device;
+  if (const char* e = std::getenv("VO_PASS2")) ctx->force_sweep = std::string(e) == "sweep";
   int rc = guarded(ctx, [&] {
     int n = 0;
     check(ctx, hipGetDeviceCount(&n), "hipGetDeviceCount");

@@ -119,7 +119,7 @@ __device__ int pass1_step(S& sm, int& cur, int n, int k, int rint, const T* z, i
     MergeScratch<T> ms{sm.ev, sm.ec, sm.tlo, sm.thi, sm.tcode, S::ECAP};
     n = merge_max(A, n, B, nb, O, cap, ms);
     if (n < 0) {
-      raise_status(status, kStatusListOverflow);
+      raise_status(status, kStatusP1List);
       return -1;
     }
     cur ^= 1;
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(kWave) void pass1_kernel(Pass1Args<T> g) {
     MergeScratch<T> ms{sm.ev, sm.ec, sm.tlo, sm.thi, sm.tcode, S::ECAP};
     const int nu = merge_max(A, fn, B, n, U, 2 * CAP, ms);
     if (nu < 0 || opos + nu > g.arena) {
-      raise_status(g.status, nu < 0 ? kStatusListOverflow : kStatusArenaOverflow);
+      raise_status(g.status, nu < 0 ? kStatusP1Union : kStatusArenaOverflow);
       return;
     }
     for (int i = L; i < nu; i += kWave) {
@@ -262,6 +262,11 @@ struct Pass2Args {
   int64_t* o_start;  // per column
   int64_t* o_cnt;    // per column, in endpoints
   int* status;
+  // list mode: recompute only the gather tiles listed in tiles[1 .. tiles[0]]
+  // (kGatherTile columns each, split into chunks of `chunk` columns), with
+  // arenas from o_base on
+  const int* tiles;
+  int64_t o_base;
 };
 
 template <class T, int SCAP, int PCAP, int NCAP, int OCAP, int FARENA, int CMAX>
@@ -308,14 +313,15 @@ struct Pass2Smem {
   int fcnt[CMAX];
 };
 
-// Union of two z-sorted disjoint interval lists (touching ones fuse).
-// Writes to (olo, ohi) and returns the count, -1 on overflow.
+// Union of two z-sorted disjoint interval lists (touching ones fuse), merged
+// through (mlo, mhi) of capacity cap_m.  Writes at most cap_o intervals to
+// (olo, ohi) and returns the count, -1 on overflow.
 __device__ int union_lists(const double* alo, const double* ahi, int na, const double* blo,
-                           const double* bhi, int nb, double* mlo, double* mhi, double* olo,
-                           double* ohi, int cap) {
+                           const double* bhi, int nb, double* mlo, double* mhi, int cap_m, double* olo,
+                           double* ohi, int cap_o) {
   const int L = lane_id();
   const int n = na + nb;
-  if (n > cap) return -1;
+  if (n > cap_m) return -1;
   for (int i = L; i < na; i += kWave) {
     const int r = rank_in([&](int j) { return blo[j]; }, nb, alo[i], false);
     mlo[i + r] = alo[i];
@@ -341,14 +347,14 @@ __device__ int union_lists(const double* alo, const double* ahi, int na, const d
     const bool fin = i < n && next_lo > run;
     const uint64_t ms = ballot(start), mf = ballot(fin);
     const int is = ns + count_below(ms), jf = nf + count_below(mf);
-    if (start) olo[is] = lo;
-    if (fin) ohi[jf] = run;
+    if (start && is < cap_o) olo[is] = lo;
+    if (fin && jf < cap_o) ohi[jf] = run;
     ns += popc(ms);
     nf += popc(mf);
     carry = __shfl(run, kWave - 1);
   }
   wave_sync();
-  return ns;
+  return ns > cap_o ? -1 : ns;
 }
 
 template <class T, class S>
@@ -410,7 +416,7 @@ __device__ int pass2_step(S& sm, int& cs, int& ns, int& cp, int& np, int k, int6
       MergeScratch<T> ms{sm.ev, sm.ec, sm.tlo, sm.thi, sm.tcode, S::ECAP};
       ns = merge_max(A, ns, B, nc, O, scap, ms);
       if (ns < 0) {
-        raise_status(g.status, kStatusListOverflow);
+        raise_status(g.status, kStatusP2Interiors);
         return -1;
       }
       cs ^= 1;
@@ -437,7 +443,7 @@ __device__ int pass2_step(S& sm, int& cs, int& ns, int& cp, int& np, int k, int6
     }
     wave_sync();
     if (np + nq > pcap) {
-      raise_status(g.status, kStatusListOverflow);
+      raise_status(g.status, kStatusP2Disks);
       return -1;
     }
     // merge candidates into the disk list by z (older first on ties)
@@ -613,12 +619,12 @@ __device__ int pass2_step(S& sm, int& cs, int& ns, int& cp, int& np, int k, int6
   }
   wave_sync();
   if (nsu > ocap || npu > ocap) {
-    raise_status(g.status, kStatusListOverflow);
+    raise_status(g.status, kStatusP2Slice);
     return -1;
   }
-  const int nu = union_lists(sm.ulo[0], sm.uhi[0], nsu, sm.ulo[1], sm.uhi[1], npu, sm.mlo, sm.mhi,
+  const int nu = union_lists(sm.ulo[0], sm.uhi[0], nsu, sm.ulo[1], sm.uhi[1], npu, sm.mlo, sm.mhi, 2 * ocap,
                              outlo, outhi, ocap);
-  if (nu < 0) raise_status(g.status, kStatusListOverflow);
+  if (nu < 0) raise_status(g.status, kStatusP2Slice);
   return nu;
 }
 
@@ -627,12 +633,19 @@ __global__ __launch_bounds__(kWave) void pass2_kernel(Pass2Args<T> g) {
   using S = Pass2Smem<T, SCAP, PCAP, NCAP, OCAP, FARENA, CMAX>;
   __shared__ S sm;
   const int L = lane_id();
-  const int y = blockIdx.x % g.ny;
-  const int ci = blockIdx.x / g.ny;
-  const int x0 = ci * g.chunk;
+  int y, x0;
+  if (g.tiles) {
+    const int per = kGatherTile / g.chunk;
+    const int tile = g.tiles[1 + blockIdx.x / per];
+    y = tile % g.ny;
+    x0 = (tile / g.ny) * kGatherTile + (blockIdx.x % per) * g.chunk;
+  } else {
+    y = blockIdx.x % g.ny;
+    x0 = (blockIdx.x / g.ny) * g.chunk;
+  }
   const int x1 = min(g.nx, x0 + g.chunk);
   if (x0 >= g.nx) return;
-  const int64_t obase = int64_t(blockIdx.x) * g.arena;
+  const int64_t obase = g.o_base + int64_t(blockIdx.x) * g.arena;
   if (*g.status != 0) return;  // pass 1 did not complete: its arena is not valid
   // forward sweep, k = x
   int cs = 0, ns = 0, cp = 0, np = 0, fpos = 0;
@@ -673,9 +686,9 @@ __global__ __launch_bounds__(kWave) void pass2_kernel(Pass2Args<T> g) {
     if (x >= x1) continue;
     const int fa = sm.fpos[x - x0], fn = sm.fcnt[x - x0];
     const int nr = union_lists(sm.flo + fa, sm.fhi + fa, fn, sm.ulo[2], sm.uhi[2], nu, sm.mlo, sm.mhi,
-                               sm.ulo[0], sm.uhi[0], OCAP);
+                               2 * OCAP, sm.ulo[0], sm.uhi[0], OCAP);
     if (nr < 0 || opos + 2 * nr > g.arena) {
-      raise_status(g.status, nr < 0 ? kStatusListOverflow : kStatusArenaOverflow);
+      raise_status(g.status, nr < 0 ? kStatusP2Slice : kStatusArenaOverflow);
       return;
     }
     for (int i = L; i < nr; i += kWave) {
@@ -775,6 +788,205 @@ __global__ void crop_complement_kernel(int nx, int ny, const int64_t* off, const
   if (add_lo) out[w++] = lo;
   for (int64_t i = head; i < n - tail; ++i) out[w++] = z[s + i];
   if (add_hi) out[w++] = hi;
+}
+
+}  // namespace vo
+
+// ---------------------------------------------------------------------------
+// Pass 2, gather form.  The output at column x is the union over source
+// columns x' with |x - x'| = d <= R of the pass-1 pieces dilated by
+// H = sqrt(R^2 - dy^2 - d^2): exactly what SeparatePower2D's forward and
+// backward sweeps assemble (their getLine emits [y1 - dz, y2 + dz] for every
+// live interior and point, SeparatePower2D.cpp:295-341).  One lane per output
+// column; the pieces of the tile's source window are staged in LDS once.
+// Within one source column the pieces are z-sorted, so touching dilated pieces
+// fuse into runs that are united into the lane's sorted output list with a
+// forward-moving cursor.  A tile whose window or output lists do not fit is
+// flagged and recomputed by the sweep kernel.
+// ---------------------------------------------------------------------------
+
+namespace vo {
+
+template <class T>
+struct Pass2GArgs {
+  int nx, ny;
+  int rint;
+  double R2;
+  int ntx;  // tiles per row
+  const T* m_lo;
+  const T* m_hi;
+  const uint16_t* m_dy;
+  const int64_t* m_start;
+  const int32_t* m_cnt;
+  int64_t m_total;
+  T* o_z;
+  int64_t arena;  // endpoints reserved per tile
+  int64_t* o_start;
+  int64_t* o_cnt;
+  int* tile_list;  // [0] = count, then the ids of tiles to recompute with the sweep kernel
+  int ntiles;
+  int* status;
+};
+
+
+template <class T, int POOL, int OCAP, int WMAX>
+struct Pass2GSmem {
+  T pa[POOL];
+  T pb[POOL];
+  uint8_t pdy[POOL];
+  int coff[WMAX + 1];
+  double hrow[256];
+  double olo[OCAP][kWave];
+  double ohi[OCAP][kWave];
+};
+
+// Unites [s, e] into the lane's sorted disjoint list (touching fuses).  `k`
+// is a cursor that only needs to be at or before the insertion point for a
+// sequence of increasing s; it is moved back when needed.  Returns false on
+// overflow.
+template <int OCAP>
+__device__ __forceinline__ bool lane_unite(double (*olo)[kWave], double (*ohi)[kWave], int L, int& n, int& k,
+                                           double s, double e) {
+  while (k > 0 && ohi[k - 1][L] >= s) --k;
+  while (k < n && ohi[k][L] < s) ++k;
+  if (k == n || olo[k][L] > e) {
+    if (n == OCAP) return false;
+    for (int j = n; j > k; --j) {
+      olo[j][L] = olo[j - 1][L];
+      ohi[j][L] = ohi[j - 1][L];
+    }
+    olo[k][L] = s;
+    ohi[k][L] = e;
+    ++n;
+    return true;
+  }
+  if (s < olo[k][L]) olo[k][L] = s;
+  double hi = fmax(ohi[k][L], e);
+  int j = k + 1;
+  while (j < n && olo[j][L] <= hi) {
+    hi = fmax(hi, ohi[j][L]);
+    ++j;
+  }
+  const int gone = j - k - 1;
+  if (gone > 0) {
+    for (int t = k + 1; t + gone < n; ++t) {
+      olo[t][L] = olo[t + gone][L];
+      ohi[t][L] = ohi[t + gone][L];
+    }
+    n -= gone;
+  }
+  ohi[k][L] = hi;
+  return true;
+}
+
+template <class T, int POOL, int OCAP, int WMAX>
+__global__ __launch_bounds__(kWave) void pass2_gather_kernel(Pass2GArgs<T> g) {
+  using S = Pass2GSmem<T, POOL, OCAP, WMAX>;
+  __shared__ S sm;
+  const int L = lane_id();
+  const int y = blockIdx.x % g.ny;
+  const int t = blockIdx.x / g.ny;
+  const int x0 = t * kGatherTile;
+  if (x0 >= g.nx) return;
+  if (*g.status != 0) return;  // pass 1 did not complete
+  const int rint = g.rint;
+  const int wlo = x0 - rint, wn = kGatherTile + 2 * rint;
+  const int64_t rowbase = int64_t(g.nx) * y;
+  // window column sizes -> prefix offsets into the pool
+  int carry = 0;
+  for (int base = 0; base < wn; base += kWave) {
+    const int w = base + L;
+    const int c = wlo + w;
+    int cnt = 0;
+    if (w < wn && c >= 0 && c < g.nx) cnt = g.m_cnt[rowbase + c];
+    int v = cnt;  // inclusive scan of counts
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const int o = __shfl_up(v, d);
+      if (L >= d) v += o;
+    }
+    if (w < wn) sm.coff[w + 1] = carry + v;
+    carry += __shfl(v, kWave - 1);
+  }
+  if (L == 0) sm.coff[0] = 0;
+  wave_sync();
+  const int total = sm.coff[wn];
+  if (total > POOL) {
+    if (L == 0) g.tile_list[1 + atomicAdd(g.tile_list, 1)] = blockIdx.x;
+    return;
+  }
+  // stage the window's pieces: one lane per source column
+  for (int w = L; w < wn; w += kWave) {
+    const int c = wlo + w;
+    if (c < 0 || c >= g.nx) continue;
+    const int64_t st = g.m_start[rowbase + c];
+    const int o = sm.coff[w], m = sm.coff[w + 1] - o;
+    if (st < 0 || st + m > g.m_total) {
+      raise_status(g.status, kStatusArenaOverflow);
+      continue;
+    }
+    for (int i = 0; i < m; ++i) {
+      sm.pa[o + i] = g.m_lo[st + i];
+      sm.pb[o + i] = g.m_hi[st + i];
+      sm.pdy[o + i] = uint8_t(g.m_dy[st + i]);
+    }
+  }
+  wave_sync();
+  const int x = x0 + L;
+  const bool active = x < g.nx;
+  int n = 0;
+  bool ok = true;
+  // source columns nearest first: d = 0, -1, +1, -2, +2, ...
+  for (int step = 0; step <= 2 * rint; ++step) {
+    const int ad = (step + 1) >> 1;
+    const int d = (step & 1) ? -ad : ad;
+    if (step == 0 || (step & 1)) {  // new |d|: table of H by dy
+      wave_sync();
+      for (int dy = L; dy <= rint; dy += kWave) {
+        const double v = g.R2 - double(dy) * double(dy) - double(ad) * double(ad);
+        sm.hrow[dy] = v >= 0 ? sqrt(v) : -1.0;
+      }
+      wave_sync();
+    }
+    const int c = x + d;
+    if (!active || !ok || c < 0 || c >= g.nx) continue;
+    const int w = c - wlo;
+    const int p0 = sm.coff[w], p1 = sm.coff[w + 1];
+    int k = 0;
+    bool run = false;
+    double rs = 0, re = 0;
+    for (int p = p0; p < p1; ++p) {
+      const double h = sm.hrow[sm.pdy[p]];
+      if (h < 0) continue;
+      const double s = double(sm.pa[p]) - h, e = double(sm.pb[p]) + h;
+      if (run && s <= re) {
+        rs = fmin(rs, s);
+        re = fmax(re, e);
+        continue;
+      }
+      if (run && !lane_unite<OCAP>(sm.olo, sm.ohi, L, n, k, rs, re)) {
+        ok = false;
+        break;
+      }
+      run = true;
+      rs = s;
+      re = e;
+    }
+    if (ok && run && !lane_unite<OCAP>(sm.olo, sm.ohi, L, n, k, rs, re)) ok = false;
+  }
+  if (ballot(active && !ok) != 0) {
+    if (L == 0) g.tile_list[1 + atomicAdd(g.tile_list, 1)] = blockIdx.x;
+    return;
+  }
+  if (!active) return;
+  const int64_t col = rowbase + x;
+  const int64_t ob = int64_t(blockIdx.x) * g.arena + int64_t(L) * 2 * OCAP;
+  for (int i = 0; i < n; ++i) {
+    g.o_z[ob + 2 * i] = T(sm.olo[i][L]);
+    g.o_z[ob + 2 * i + 1] = T(sm.ohi[i][L]);
+  }
+  g.o_start[col] = ob;
+  g.o_cnt[col] = 2 * n;
 }
 
 }  // namespace vo

@@ -16,6 +16,7 @@ namespace vo {
 
 constexpr int kWave = 64;
 constexpr int kGapCode = INT32_MIN;  // "no owner" code of an elementary interval
+constexpr int kGatherTile = kWave;   // output columns per tile of the gather form of pass 2
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x; }
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
@@ -27,9 +28,13 @@ __device__ __forceinline__ void wave_sync() { __syncthreads(); }
 
 // Status bits written by the kernels (host turns them into error codes).
 enum : int {
-  kStatusListOverflow = 1,    // a sweep list exceeded its LDS capacity
-  kStatusArenaOverflow = 2,   // an output arena exceeded its capacity
-  kStatusColumnOverflow = 4,  // one column held more items than a batch can stage
+  kStatusArenaOverflow = 1,   // an output arena (LDS or HBM) exceeded its capacity
+  kStatusP1List = 2,          // pass 1: live piece list
+  kStatusP1Union = 4,         // pass 1: forward/backward union
+  kStatusP2Interiors = 8,     // pass 2: interior list
+  kStatusP2Disks = 16,        // pass 2: disk list
+  kStatusP2Slice = 32,        // pass 2: intervals of one line slice
+  kStatusListOverflow = kStatusP1List | kStatusP1Union | kStatusP2Interiors | kStatusP2Disks | kStatusP2Slice,
 };
 
 __device__ __forceinline__ void raise_status(int* status, int bits) {
