@@ -1,5 +1,7 @@
-"""The sweep form of pass 2 (used whole with VO_PASS2=sweep, and for the tiles
-the gather form hands back) against the oracle, through the C ABI."""
+"""The alternative kernel forms against the oracle, through the C ABI: the sweep
+form of pass 2 (used whole with VO_PASS2=sweep, and for the tiles the gather
+form hands back) and the culling and painter forms of pass 1 (VO_PASS1=cull,
+VO_PASS1=paint); the default forms are the z-sweep pass 1 and the gather pass 2."""
 
 import os
 
@@ -13,18 +15,28 @@ from .helpers import compare_csr
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def hip_sweep():
+def _context_with(**env):
     from voroffset_amd.morpho import VoronoiMorphoHip
-    old = os.environ.get("VO_PASS2")
-    os.environ["VO_PASS2"] = "sweep"
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
     try:
-        m = VoronoiMorphoHip("fp64")
+        return VoronoiMorphoHip("fp64")
     finally:
-        if old is None:
-            del os.environ["VO_PASS2"]
-        else:
-            os.environ["VO_PASS2"] = old
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+
+
+FORMS = {"sweep-pass2": {"VO_PASS2": "sweep"}, "cull-pass1": {"VO_PASS1": "cull"},
+         "paint-pass1": {"VO_PASS1": "paint"}}
+
+
+@pytest.fixture(scope="module", params=sorted(FORMS))
+def hip_sweep(request):
+    env = FORMS[request.param]
+    m = _context_with(**env)
     yield m
     m.close()
 

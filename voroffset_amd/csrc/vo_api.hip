@@ -31,8 +31,10 @@ namespace {
 // Kernel shapes.  The LDS footprint of each instantiation is checked below.
 constexpr int kP1Cap = 128, kP1NCap = 64;
 constexpr int kP2SCap = 128, kP2PCap = 256, kP2NCap = 64, kP2OCap = 256;
-constexpr int kGOCap = 16;    // gather path: intervals per output column list
-constexpr int kGWMax = 64 + 2 * 128;  // gather path: widest source window (R <= 128)
+constexpr int kGOCap = 16;            // gather form: intervals per output column list
+constexpr int kGWMax = 64 + 2 * 128;  // gather form: widest source window (R < 129)
+constexpr int kGHT = 40;              // gather form: H table up to this radius
+constexpr int kZECap = 3072, kZNB = 1024, kZWMax = 64 + 2 * 128;  // pass-1 z-sweep form
 
 template <class T>
 struct Shapes;
@@ -40,13 +42,11 @@ template <>
 struct Shapes<float> {
   static constexpr int p1_arena = 2048, p1_cmax = 64;
   static constexpr int p2_farena = 512, p2_cmax = 64;
-  static constexpr int g_pool = 3072;
 };
 template <>
 struct Shapes<double> {
   static constexpr int p1_arena = 1024, p1_cmax = 32;
   static constexpr int p2_farena = 256, p2_cmax = 32;
-  static constexpr int g_pool = 1536;
 };
 
 template <class T>
@@ -55,12 +55,14 @@ template <class T>
 using P2S = vo::Pass2Smem<T, kP2SCap, kP2PCap, kP2NCap, kP2OCap, Shapes<T>::p2_farena, Shapes<T>::p2_cmax>;
 static_assert(sizeof(P1S<float>) <= 65536, "pass-1 LDS (fp32)");
 static_assert(sizeof(P1S<double>) <= 65536, "pass-1 LDS (fp64)");
+template <class T>
+using P1P = vo::Pass1PSmem<T, kP1Cap, Shapes<T>::p1_arena, Shapes<T>::p1_cmax>;
+static_assert(sizeof(P1P<float>) <= 65536, "pass-1 painter LDS (fp32)");
+static_assert(sizeof(P1P<double>) <= 65536, "pass-1 painter LDS (fp64)");
 static_assert(sizeof(P2S<float>) <= 65536, "pass-2 LDS (fp32)");
 static_assert(sizeof(P2S<double>) <= 65536, "pass-2 LDS (fp64)");
-template <class T>
-using P2G = vo::Pass2GSmem<T, Shapes<T>::g_pool, kGOCap, kGWMax>;
-static_assert(sizeof(P2G<float>) <= 65536, "pass-2 gather LDS (fp32)");
-static_assert(sizeof(P2G<double>) <= 65536, "pass-2 gather LDS (fp64)");
+static_assert(sizeof(vo::Pass2GSmem<kGOCap, kGWMax, kGHT>) <= 65536, "pass-2 gather LDS");
+static_assert(sizeof(vo::Pass1ZSmem<double, kZECap, kZNB, kZWMax>) <= 65536, "pass-1 z-sweep LDS");
 
 struct Buffer {
   void* ptr = nullptr;
@@ -68,7 +70,7 @@ struct Buffer {
 };
 
 enum Slot {
-  kM_LO, kM_HI, kM_DY, kM_START, kM_CNT,  // pass-1 output
+  kM_LO, kM_HI, kM_DY, kM_CNT,  // pass-1 output
   kO_Z, kO_START, kO_LEN,                  // pass-2 output
   kSTATUS, kSCAN_TMP, kTILES,
   kV0_OFF, kV0_Z, kV1_OFF, kV1_Z, kV2_OFF, kV2_Z, kV3_OFF, kV3_Z,  // device volumes
@@ -93,9 +95,10 @@ struct vo_context {
   int64_t* host_words = nullptr;  // pinned: status + counts
   std::string error;
   int64_t last_triples = 0, last_outputs = 0;
-  int p1_cap_per_col = 64;  // pass-1 arena triples per column (grows on overflow)
+  int p1_caps = 96;         // pass-1 slots per column (grows on overflow)
   int p2_cap_per_col = 16;  // pass-2 arena endpoints per column (grows on overflow)
   bool force_sweep = false;  // VO_PASS2=sweep: always use the sweep form of pass 2
+  int pass1_form = 0;        // VO_PASS1: 0 = z-sweep (default), 1 = "paint", 2 = "cull"
   int last_fallback_tiles = 0;
 };
 
@@ -183,7 +186,6 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
   const int rint = int(std::floor(R));
   const double R2 = R * R;
   int* status = static_cast<int*>(ws(ctx, kSTATUS, 64));
-  int64_t* m_start = static_cast<int64_t*>(ws(ctx, kM_START, ncols * sizeof(int64_t)));
   int32_t* m_cnt = static_cast<int32_t*>(ws(ctx, kM_CNT, ncols * sizeof(int32_t)));
   int64_t* o_start = static_cast<int64_t*>(ws(ctx, kO_START, ncols * sizeof(int64_t)));
   int64_t* o_len = static_cast<int64_t*>(ws(ctx, kO_LEN, (ncols + 1) * sizeof(int64_t)));
@@ -199,13 +201,15 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
   }
   int chunk1 = std::min(Shapes<T>::p1_cmax, 32);
   int chunk2 = Shapes<T>::p2_cmax;
+  bool zsweep_failed = false;  // a window too dense for the z-sweep form: sweep along y instead
   for (int attempt = 0;; ++attempt) {
     const int nch1 = (ny + chunk1 - 1) / chunk1;
-    const int64_t arena1 = int64_t(chunk1) * ctx->p1_cap_per_col;
     const int64_t grid1 = int64_t(nx) * nch1;
-    T* m_lo = static_cast<T*>(ws(ctx, kM_LO, grid1 * arena1 * sizeof(T)));
-    T* m_hi = static_cast<T*>(ws(ctx, kM_HI, grid1 * arena1 * sizeof(T)));
-    uint16_t* m_dy = static_cast<uint16_t*>(ws(ctx, kM_DY, grid1 * arena1 * sizeof(uint16_t)));
+    const int caps = ctx->p1_caps;
+    const int64_t slots = ncols * caps;
+    T* m_lo = static_cast<T*>(ws(ctx, kM_LO, slots * sizeof(T)));
+    T* m_hi = static_cast<T*>(ws(ctx, kM_HI, slots * sizeof(T)));
+    uint16_t* m_dy = static_cast<uint16_t*>(ws(ctx, kM_DY, slots * sizeof(uint16_t)));
     const int ntx = (nx + vo::kGatherTile - 1) / vo::kGatherTile;
     const int64_t ntiles = int64_t(ny) * ntx;
     const int64_t garena = int64_t(vo::kGatherTile) * 2 * kGOCap;  // endpoints per gather tile
@@ -219,18 +223,30 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
     if (grid1 > INT32_MAX || fb_chunks > INT32_MAX) fail(ctx, VO_ERR_ARGUMENT, "grid too large");
 
     vo::Pass1Args<T> a1{nx, ny, rint, chunk1, nch1, in.off, static_cast<const T*>(in.z), m_lo, m_hi, m_dy,
-                        arena1, m_start, m_cnt, status};
+                        caps, m_cnt, status};
     check(ctx, hipEventRecord(ctx->ev[0], ctx->stream), "event");
-    hipLaunchKernelGGL((vo::pass1_kernel<T, kP1Cap, kP1NCap, Shapes<T>::p1_arena, Shapes<T>::p1_cmax>),
-                       dim3(unsigned(grid1)), dim3(vo::kWave), 0, ctx->stream, a1);
+    const bool zsweep = ctx->pass1_form == 0 && rint <= 128 && !zsweep_failed;
+    if (zsweep) {
+      vo::Pass1ZArgs<T> az{nx, ny, rint, in.off, static_cast<const T*>(in.z), m_lo, m_hi, m_dy, caps, m_cnt,
+                           status};
+      const int64_t gridz = int64_t(nx) * ((ny + vo::kZTile - 1) / vo::kZTile);
+      hipLaunchKernelGGL((vo::pass1_zsweep_kernel<T, kZECap, kZNB, kZWMax>), dim3(unsigned(gridz)),
+                         dim3(vo::kWave), 0, ctx->stream, az);
+    } else if (ctx->pass1_form == 2) {
+      hipLaunchKernelGGL((vo::pass1_kernel<T, kP1Cap, kP1NCap, Shapes<T>::p1_arena, Shapes<T>::p1_cmax>),
+                         dim3(unsigned(grid1)), dim3(vo::kWave), 0, ctx->stream, a1);
+    } else {
+      hipLaunchKernelGGL((vo::pass1_paint_kernel<T, kP1Cap, Shapes<T>::p1_arena, Shapes<T>::p1_cmax>),
+                         dim3(unsigned(grid1)), dim3(vo::kWave), 0, ctx->stream, a1);
+    }
     check(ctx, hipGetLastError(), "pass1 launch");
     check(ctx, hipEventRecord(ctx->ev[1], ctx->stream), "event");
     const bool gather = rint <= 128 && !ctx->force_sweep;
     int nfall = 0;
     if (gather) {
-      vo::Pass2GArgs<T> ag{nx, ny, rint, R2, ntx, m_lo, m_hi, m_dy, m_start, m_cnt, grid1 * arena1,
-                           o_z, garena, o_start, o_len, tiles, int(ntiles), status};
-      hipLaunchKernelGGL((vo::pass2_gather_kernel<T, Shapes<T>::g_pool, kGOCap, kGWMax>), dim3(unsigned(ntiles)),
+      vo::Pass2GArgs<T> ag{nx, ny, rint, R2, m_lo, m_hi, m_dy, caps, m_cnt, o_z, garena, o_start, o_len,
+                           tiles, status};
+      hipLaunchKernelGGL((vo::pass2_gather_kernel<T, kGOCap, kGWMax, kGHT>), dim3(unsigned(ntiles)),
                          dim3(vo::kWave), 0, ctx->stream, ag);
       check(ctx, hipGetLastError(), "pass2 gather launch");
       nfall = int(read_word32(ctx, tiles));
@@ -238,8 +254,8 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
     // the sweep form: the whole volume, or the tiles the gather form gave back
     const int64_t grid2 = gather ? int64_t(nfall) * (vo::kGatherTile / chunk2) : int64_t(ny) * ((nx + chunk2 - 1) / chunk2);
     if (grid2 > 0) {
-      vo::Pass2Args<T> a2{nx, ny, rint, R2, chunk2, 0, m_lo, m_hi, m_dy, m_start, m_cnt, grid1 * arena1,
-                          o_z, arena2, o_start, o_len, status, gather ? tiles : nullptr, ntiles * garena};
+      vo::Pass2Args<T> a2{nx, ny, rint, R2, chunk2, 0, m_lo, m_hi, m_dy, caps, m_cnt, o_z, arena2, o_start,
+                          o_len, status, gather ? tiles : nullptr, ntiles * garena};
       hipLaunchKernelGGL((vo::pass2_kernel<T, kP2SCap, kP2PCap, kP2NCap, kP2OCap, Shapes<T>::p2_farena,
                                            Shapes<T>::p2_cmax>),
                          dim3(unsigned(grid2)), dim3(vo::kWave), 0, ctx->stream, a2);
@@ -262,12 +278,18 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
       }
       break;
     }
+    if ((st & vo::kStatusP1List) && ctx->pass1_form == 0 && !zsweep_failed) {
+      zsweep_failed = true;
+      continue;
+    }
     if (st & vo::kStatusListOverflow) fail(ctx, VO_ERR_CAPACITY, status_text(st));
     // arena overflow: give each column more room (LDS arenas: shorter chunks)
     if (attempt >= 6) fail(ctx, VO_ERR_CAPACITY, "arena overflow after retries");
+    // (the pass-1 forward arena in LDS wants shorter chunks, the slot
+    // layouts and the pass-2 arena want more room per column)
     if (chunk1 > 8) chunk1 /= 2;
     if (chunk2 > 8) chunk2 /= 2;
-    ctx->p1_cap_per_col *= 2;
+    ctx->p1_caps *= 2;
     ctx->p2_cap_per_col *= 2;
   }
   // output CSR
@@ -438,6 +460,8 @@ int vo_create(vo_context** out, int device) {
   auto* ctx = new vo_context;
   ctx->device = device;
   if (const char* e = std::getenv("VO_PASS2")) ctx->force_sweep = std::string(e) == "sweep";
+  if (const char* e = std::getenv("VO_PASS1"))
+    ctx->pass1_form = std::string(e) == "cull" ? 2 : (std::string(e) == "paint" ? 1 : 0);
   int rc = guarded(ctx, [&] {
     int n = 0;
     check(ctx, hipGetDeviceCount(&n), "hipGetDeviceCount");

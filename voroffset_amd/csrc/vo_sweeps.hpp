@@ -45,14 +45,19 @@ struct Pass1Args {
   int nchunks;   // ceil(ny / chunk)
   const int64_t* in_off;
   const T* in_z;
+  // output triples in slot layout: triple j of column (x, y) at
+  // (y * caps + j) * nx + x, so that a row's j-th triples are contiguous
   T* m_lo;
   T* m_hi;
   uint16_t* m_dy;
-  int64_t arena;     // triples reserved per chunk in the output arena
-  int64_t* m_start;  // per column: first triple in the arena
-  int32_t* m_cnt;    // per column: number of triples
+  int caps;        // slots per column
+  int32_t* m_cnt;  // per column: number of triples
   int* status;
 };
+
+__device__ __forceinline__ int64_t slot_index(int nx, int caps, int x, int y, int j) {
+  return (int64_t(y) * caps + j) * nx + x;
+}
 
 template <class T, int CAP, int NCAP, int ARENA, int CMAX>
 struct Pass1Smem {
@@ -171,7 +176,6 @@ __global__ __launch_bounds__(kWave) void pass1_kernel(Pass1Args<T> g) {
   const int y0 = ci * g.chunk;
   const int y1 = min(g.ny, y0 + g.chunk);
   if (y0 >= g.ny) return;
-  const int64_t obase = int64_t(blockIdx.x) * g.arena;
   // forward sweep, k = y
   int cur = 0, n = 0, apos = 0;
   bool ok = true;
@@ -206,7 +210,6 @@ __global__ __launch_bounds__(kWave) void pass1_kernel(Pass1Args<T> g) {
   // backward sweep, k = ny-1-y; unite with the forward result and store
   cur = 0;
   n = 0;
-  int64_t opos = 0;
   for (int y = min(g.ny - 1, y1 - 1 + g.rint); y >= y0; --y) {
     const int k = g.ny - 1 - y;
     const int64_t col = int64_t(x) + int64_t(g.nx) * y;
@@ -223,20 +226,17 @@ __global__ __launch_bounds__(kWave) void pass1_kernel(Pass1Args<T> g) {
     PieceRef<T> U{sm.ulo, sm.uhi, sm.ukey};
     MergeScratch<T> ms{sm.ev, sm.ec, sm.tlo, sm.thi, sm.tcode, S::ECAP};
     const int nu = merge_max(A, fn, B, n, U, 2 * CAP, ms);
-    if (nu < 0 || opos + nu > g.arena) {
+    if (nu < 0 || nu > g.caps) {
       raise_status(g.status, nu < 0 ? kStatusP1Union : kStatusArenaOverflow);
       return;
     }
     for (int i = L; i < nu; i += kWave) {
-      g.m_lo[obase + opos + i] = sm.ulo[i];
-      g.m_hi[obase + opos + i] = sm.uhi[i];
-      g.m_dy[obase + opos + i] = uint16_t(-sm.ukey[i]);
+      const int64_t o = slot_index(g.nx, g.caps, x, y, i);
+      g.m_lo[o] = sm.ulo[i];
+      g.m_hi[o] = sm.uhi[i];
+      g.m_dy[o] = uint16_t(-sm.ukey[i]);
     }
-    if (L == 0) {
-      g.m_start[col] = obase + opos;
-      g.m_cnt[col] = nu;
-    }
-    opos += nu;
+    if (L == 0) g.m_cnt[col] = nu;
   }
 }
 
@@ -251,12 +251,11 @@ struct Pass2Args {
   double R2;
   int chunk;
   int nchunks;
-  const T* m_lo;
+  const T* m_lo;  // pass-1 triples, slot layout (see Pass1Args)
   const T* m_hi;
   const uint16_t* m_dy;
-  const int64_t* m_start;
+  int caps;
   const int32_t* m_cnt;
-  int64_t m_total;   // capacity of the pass-1 arena (bounds every read)
   T* o_z;            // output arena (endpoints)
   int64_t arena;     // endpoints reserved per chunk
   int64_t* o_start;  // per column
@@ -358,7 +357,7 @@ __device__ int union_lists(const double* alo, const double* ahi, int na, const d
 }
 
 template <class T, class S>
-__device__ int pass2_step(S& sm, int& cs, int& ns, int& cp, int& np, int k, int64_t mstart, int mcnt,
+__device__ int pass2_step(S& sm, int& cs, int& ns, int& cp, int& np, int k, int xcol, int ycol, int mcnt,
                           const Pass2Args<T>& g, int ncap, int scap, int pcap, int ocap,
                           double* outlo, double* outhi) {
   const int L = lane_id();
@@ -385,16 +384,17 @@ __device__ int pass2_step(S& sm, int& cs, int& ns, int& cp, int& np, int k, int6
     cp ^= 1;
   }
   // insert the column's triples, a batch at a time
-  if (mstart < 0 || mcnt < 0 || mstart + mcnt > g.m_total) {
+  if (mcnt < 0 || mcnt > g.caps) {
     raise_status(g.status, kStatusArenaOverflow);
     return -1;
   }
   for (int b0 = 0; b0 < mcnt; b0 += ncap) {
     const int nb = min(ncap, mcnt - b0);
     for (int i = L; i < nb; i += kWave) {
-      const int dy = g.m_dy[mstart + b0 + i];
-      sm.nlo[i] = g.m_lo[mstart + b0 + i];
-      sm.nhi[i] = g.m_hi[mstart + b0 + i];
+      const int64_t o = slot_index(g.nx, g.caps, xcol, ycol, b0 + i);
+      const int dy = g.m_dy[o];
+      sm.nlo[i] = g.m_lo[o];
+      sm.nhi[i] = g.m_hi[o];
       sm.ndy[i] = dy;
       const int t = reach_of(dy, g.R2);
       sm.nD[i] = t < 0 ? k - 1 : k + t;
@@ -651,8 +651,8 @@ __global__ __launch_bounds__(kWave) void pass2_kernel(Pass2Args<T> g) {
   int cs = 0, ns = 0, cp = 0, np = 0, fpos = 0;
   for (int x = max(0, x0 - g.rint); x < x1; ++x) {
     const int64_t col = int64_t(x) + int64_t(g.nx) * y;
-    const int nu = pass2_step<T>(sm, cs, ns, cp, np, x, g.m_start[col], g.m_cnt[col], g, NCAP, SCAP,
-                                 PCAP, OCAP, sm.ulo[2], sm.uhi[2]);
+    const int nu = pass2_step<T>(sm, cs, ns, cp, np, x, x, y, g.m_cnt[col], g, NCAP, SCAP, PCAP, OCAP,
+                                 sm.ulo[2], sm.uhi[2]);
     if (nu < 0) return;
     if (x >= x0) {
       if (fpos + nu > FARENA) {
@@ -680,8 +680,8 @@ __global__ __launch_bounds__(kWave) void pass2_kernel(Pass2Args<T> g) {
   for (int x = min(g.nx - 1, x1 - 1 + g.rint); x >= x0; --x) {
     const int k = g.nx - 1 - x;
     const int64_t col = int64_t(x) + int64_t(g.nx) * y;
-    const int nu = pass2_step<T>(sm, cs, ns, cp, np, k, g.m_start[col], g.m_cnt[col], g, NCAP, SCAP,
-                                 PCAP, OCAP, sm.ulo[2], sm.uhi[2]);
+    const int nu = pass2_step<T>(sm, cs, ns, cp, np, k, x, y, g.m_cnt[col], g, NCAP, SCAP, PCAP, OCAP,
+                                 sm.ulo[2], sm.uhi[2]);
     if (nu < 0) return;
     if (x >= x1) continue;
     const int fa = sm.fpos[x - x0], fn = sm.fcnt[x - x0];
@@ -798,11 +798,11 @@ __global__ void crop_complement_kernel(int nx, int ny, const int64_t* off, const
 // H = sqrt(R^2 - dy^2 - d^2): exactly what SeparatePower2D's forward and
 // backward sweeps assemble (their getLine emits [y1 - dz, y2 + dz] for every
 // live interior and point, SeparatePower2D.cpp:295-341).  One lane per output
-// column; the pieces of the tile's source window are staged in LDS once.
-// Within one source column the pieces are z-sorted, so touching dilated pieces
-// fuse into runs that are united into the lane's sorted output list with a
-// forward-moving cursor.  A tile whose window or output lists do not fit is
-// flagged and recomputed by the sweep kernel.
+// column.  Piece j of the 2R+1 source columns of every lane is read with one
+// coalesced load per d (the slot layout keeps a row's j-th pieces
+// contiguous), dilated with H from an LDS table, and united into the lane's
+// sorted output list in LDS; a cached covering interval skips the pieces that
+// change nothing.  A tile whose lists overflow is recomputed by the sweep form.
 // ---------------------------------------------------------------------------
 
 namespace vo {
@@ -812,38 +812,30 @@ struct Pass2GArgs {
   int nx, ny;
   int rint;
   double R2;
-  int ntx;  // tiles per row
   const T* m_lo;
   const T* m_hi;
   const uint16_t* m_dy;
-  const int64_t* m_start;
+  int caps;
   const int32_t* m_cnt;
-  int64_t m_total;
   T* o_z;
   int64_t arena;  // endpoints reserved per tile
   int64_t* o_start;
   int64_t* o_cnt;
-  int* tile_list;  // [0] = count, then the ids of tiles to recompute with the sweep kernel
-  int ntiles;
+  int* tile_list;  // [0] = count, then the ids of tiles to recompute with the sweep form
   int* status;
 };
 
-
-template <class T, int POOL, int OCAP, int WMAX>
+template <int OCAP, int WMAX, int HT>
 struct Pass2GSmem {
-  T pa[POOL];
-  T pb[POOL];
-  uint8_t pdy[POOL];
-  int coff[WMAX + 1];
-  double hrow[256];
+  int cnt[WMAX];
+  double h[(HT + 1) * (HT + 1)];
   double olo[OCAP][kWave];
   double ohi[OCAP][kWave];
 };
 
-// Unites [s, e] into the lane's sorted disjoint list (touching fuses).  `k`
-// is a cursor that only needs to be at or before the insertion point for a
-// sequence of increasing s; it is moved back when needed.  Returns false on
-// overflow.
+// Unites [s, e] into the lane's sorted disjoint list (touching fuses),
+// starting the search at cursor k; on return k indexes the interval that now
+// holds [s, e].  Returns false on overflow.
 template <int OCAP>
 __device__ __forceinline__ bool lane_unite(double (*olo)[kWave], double (*ohi)[kWave], int L, int& n, int& k,
                                            double s, double e) {
@@ -879,56 +871,34 @@ __device__ __forceinline__ bool lane_unite(double (*olo)[kWave], double (*ohi)[k
   return true;
 }
 
-template <class T, int POOL, int OCAP, int WMAX>
+template <class T, int OCAP, int WMAX, int HT>
 __global__ __launch_bounds__(kWave) void pass2_gather_kernel(Pass2GArgs<T> g) {
-  using S = Pass2GSmem<T, POOL, OCAP, WMAX>;
+  using S = Pass2GSmem<OCAP, WMAX, HT>;
   __shared__ S sm;
   const int L = lane_id();
   const int y = blockIdx.x % g.ny;
-  const int t = blockIdx.x / g.ny;
-  const int x0 = t * kGatherTile;
+  const int x0 = (blockIdx.x / g.ny) * kGatherTile;
   if (x0 >= g.nx) return;
   if (*g.status != 0) return;  // pass 1 did not complete
   const int rint = g.rint;
   const int wlo = x0 - rint, wn = kGatherTile + 2 * rint;
   const int64_t rowbase = int64_t(g.nx) * y;
-  // window column sizes -> prefix offsets into the pool
-  int carry = 0;
-  for (int base = 0; base < wn; base += kWave) {
-    const int w = base + L;
-    const int c = wlo + w;
-    int cnt = 0;
-    if (w < wn && c >= 0 && c < g.nx) cnt = g.m_cnt[rowbase + c];
-    int v = cnt;  // inclusive scan of counts
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-      const int o = __shfl_up(v, d);
-      if (L >= d) v += o;
-    }
-    if (w < wn) sm.coff[w + 1] = carry + v;
-    carry += __shfl(v, kWave - 1);
-  }
-  if (L == 0) sm.coff[0] = 0;
-  wave_sync();
-  const int total = sm.coff[wn];
-  if (total > POOL) {
-    if (L == 0) g.tile_list[1 + atomicAdd(g.tile_list, 1)] = blockIdx.x;
-    return;
-  }
-  // stage the window's pieces: one lane per source column
+  int jmax = 0;
   for (int w = L; w < wn; w += kWave) {
     const int c = wlo + w;
-    if (c < 0 || c >= g.nx) continue;
-    const int64_t st = g.m_start[rowbase + c];
-    const int o = sm.coff[w], m = sm.coff[w + 1] - o;
-    if (st < 0 || st + m > g.m_total) {
-      raise_status(g.status, kStatusArenaOverflow);
-      continue;
-    }
-    for (int i = 0; i < m; ++i) {
-      sm.pa[o + i] = g.m_lo[st + i];
-      sm.pb[o + i] = g.m_hi[st + i];
-      sm.pdy[o + i] = uint8_t(g.m_dy[st + i]);
+    const int m = (c >= 0 && c < g.nx) ? min(g.m_cnt[rowbase + c], g.caps) : 0;
+    sm.cnt[w] = m;
+    jmax = max(jmax, m);
+  }
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) jmax = max(jmax, __shfl_xor(jmax, d));
+  const bool table = rint <= HT;
+  if (table) {
+    const int w1 = rint + 1;
+    for (int i = L; i < w1 * w1; i += kWave) {
+      const int dy = i / w1, ad = i % w1;
+      const double v = g.R2 - double(dy) * double(dy) - double(ad) * double(ad);
+      sm.h[i] = v >= 0 ? sqrt(v) : -1.0;
     }
   }
   wave_sync();
@@ -936,43 +906,51 @@ __global__ __launch_bounds__(kWave) void pass2_gather_kernel(Pass2GArgs<T> g) {
   const bool active = x < g.nx;
   int n = 0;
   bool ok = true;
-  // source columns nearest first: d = 0, -1, +1, -2, +2, ...
-  for (int step = 0; step <= 2 * rint; ++step) {
-    const int ad = (step + 1) >> 1;
-    const int d = (step & 1) ? -ad : ad;
-    if (step == 0 || (step & 1)) {  // new |d|: table of H by dy
-      wave_sync();
-      for (int dy = L; dy <= rint; dy += kWave) {
-        const double v = g.R2 - double(dy) * double(dy) - double(ad) * double(ad);
-        sm.hrow[dy] = v >= 0 ? sqrt(v) : -1.0;
+  double clo = INFINITY, chi = -INFINITY;  // an interval known to lie inside the union
+  constexpr int B = 8;  // source columns fetched together (independent loads in flight)
+  for (int j = 0; j < jmax; ++j) {
+    const int64_t srow = (int64_t(y) * g.caps + j) * g.nx;
+    for (int d0 = -rint; d0 <= rint; d0 += B) {
+      T va[B], vb[B];
+      int vdy[B];
+      bool live[B];
+#pragma unroll
+      for (int u = 0; u < B; ++u) {
+        const int d = d0 + u, c = x + d;
+        live[u] = active && d <= rint && c >= 0 && c < g.nx && j < sm.cnt[L + d + rint];
+        va[u] = vb[u] = T(0);
+        vdy[u] = 0;
+        if (live[u]) {
+          va[u] = g.m_lo[srow + c];
+          vb[u] = g.m_hi[srow + c];
+          vdy[u] = g.m_dy[srow + c];
+        }
       }
-      wave_sync();
+#pragma unroll
+      for (int u = 0; u < B; ++u) {
+        if (!live[u] || !ok) continue;
+        const int d = d0 + u;
+        const int ad = d < 0 ? -d : d;
+        const int dy = vdy[u];
+        double h;
+        if (table) {
+          h = dy <= rint ? sm.h[dy * (rint + 1) + ad] : -1.0;
+        } else {
+          const double v = g.R2 - double(dy) * double(dy) - double(ad) * double(ad);
+          h = v >= 0 ? sqrt(v) : -1.0;
+        }
+        if (h < 0) continue;
+        const double s = double(va[u]) - h, e = double(vb[u]) + h;
+        if (s >= clo && e <= chi) continue;
+        int k = 0;
+        if (!lane_unite<OCAP>(sm.olo, sm.ohi, L, n, k, s, e)) {
+          ok = false;
+          continue;
+        }
+        clo = sm.olo[k][L];
+        chi = sm.ohi[k][L];
+      }
     }
-    const int c = x + d;
-    if (!active || !ok || c < 0 || c >= g.nx) continue;
-    const int w = c - wlo;
-    const int p0 = sm.coff[w], p1 = sm.coff[w + 1];
-    int k = 0;
-    bool run = false;
-    double rs = 0, re = 0;
-    for (int p = p0; p < p1; ++p) {
-      const double h = sm.hrow[sm.pdy[p]];
-      if (h < 0) continue;
-      const double s = double(sm.pa[p]) - h, e = double(sm.pb[p]) + h;
-      if (run && s <= re) {
-        rs = fmin(rs, s);
-        re = fmax(re, e);
-        continue;
-      }
-      if (run && !lane_unite<OCAP>(sm.olo, sm.ohi, L, n, k, rs, re)) {
-        ok = false;
-        break;
-      }
-      run = true;
-      rs = s;
-      re = e;
-    }
-    if (ok && run && !lane_unite<OCAP>(sm.olo, sm.ohi, L, n, k, rs, re)) ok = false;
   }
   if (ballot(active && !ok) != 0) {
     if (L == 0) g.tile_list[1 + atomicAdd(g.tile_list, 1)] = blockIdx.x;
@@ -987,6 +965,623 @@ __global__ __launch_bounds__(kWave) void pass2_gather_kernel(Pass2GArgs<T> g) {
   }
   g.o_start[col] = ob;
   g.o_cnt[col] = 2 * n;
+}
+
+}  // namespace vo
+
+// ---------------------------------------------------------------------------
+// Pass 1, painter form.  The same per-line sweep as pass1_kernel without the
+// crossing-point culling: the live list holds, for every z, the nearest line
+// (at most rint behind) whose input covers z, i.e. the newest piece wins.  A
+// step subtracts the new column's segments from every live piece (lanes over
+// pieces, segments broadcast from registers), places the surviving fragments
+// and the new segments by prefix counts, and drops pieces older than rint.
+// Culling only thins the intermediate (it never changes the dilation), and on
+// Poisson volumes it removes under 15% of the triples, so this form skips it.
+// ---------------------------------------------------------------------------
+
+namespace vo {
+
+__device__ __forceinline__ float lane_bcast(float v, int j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ __forceinline__ double lane_bcast(double v, int j) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(int(b & 0xffffffff), j);
+  const int hi = __builtin_amdgcn_readlane(int(b >> 32), j);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | (static_cast<unsigned int>(lo)));
+}
+
+template <class T, int CAP, int ARENA, int CMAX>
+struct Pass1PSmem {
+  static constexpr int ECAP = 4 * CAP;
+  T lo[2][CAP];
+  T hi[2][CAP];
+  int key[2][CAP];
+  int hist[kWave + 1];
+  // forward results of the chunk
+  T alo[ARENA];
+  T ahi[ARENA];
+  uint16_t ady[ARENA];
+  int apos[CMAX];
+  int acnt[CMAX];
+  // union of the forward and backward results
+  T ev[ECAP];
+  int ec[ECAP];
+  T tlo[ECAP];
+  T thi[ECAP];
+  int tcode[ECAP];
+  int ka[CAP];
+  int kb[CAP];
+  T ulo[2 * CAP];
+  T uhi[2 * CAP];
+  int ukey[2 * CAP];
+};
+
+// Fragments of piece [lo, hi] (lo < hi) left by the batch's sorted disjoint
+// segments (lane j holds segment j, j < nb).  Calls emit(flo, fhi, below) for
+// each, where `below` is the number of batch segments under the fragment.
+template <class T, class Emit>
+__device__ __forceinline__ void piece_fragments(double lo, double hi, T nlo, T nhi, int nb, Emit&& emit) {
+  double c = lo;
+  bool done = false;
+  for (int j = 0; j < nb; ++j) {
+    const double a = lane_bcast(nlo, j), b = lane_bcast(nhi, j);
+    if (done || b <= c || a >= hi) continue;
+    if (a > c) emit(c, a, j);
+    c = fmax(c, b);
+    done = c >= hi;
+  }
+  if (!done) {
+    int below = 0;
+    for (int j = 0; j < nb; ++j) below += double(lane_bcast(nlo, j)) < hi ? 1 : 0;
+    emit(c, hi, below);
+  }
+}
+
+// A point piece survives unless a batch segment covers it; returns the number
+// of batch segments below it, or -1 when covered.
+template <class T>
+__device__ __forceinline__ int point_below(double z, T nlo, T nhi, int nb) {
+  int below = 0;
+  bool covered = false;
+  for (int j = 0; j < nb; ++j) {
+    const double a = lane_bcast(nlo, j), b = lane_bcast(nhi, j);
+    covered |= a <= z && z <= b;
+    below += b < z ? 1 : 0;
+  }
+  return covered ? -1 : below;
+}
+
+// One painter step on line k; the column's segments are z[zoff .. zoff+2*np),
+// the first batch of which is already in (pre_lo, pre_hi).  Returns the new
+// size of list `cur` (which flips), or -1 on overflow.
+template <class T, class S>
+__device__ int paint_step(S& sm, int& cur, int n, int k, int rint, const T* z, int64_t zoff, int64_t np,
+                          T pre_lo, T pre_hi, int cap, int* status) {
+  const int L = lane_id();
+  for (int64_t b0 = 0; b0 == 0 || b0 < np; b0 += kWave) {
+    const int nb = int((np - b0) < kWave ? (np - b0) : int64_t(kWave));
+    T nlo = pre_lo, nhi = pre_hi;
+    if (b0 > 0 && L < nb) {
+      nlo = z[zoff + 2 * (b0 + L)];
+      nhi = z[zoff + 2 * (b0 + L) + 1];
+    }
+    if (L <= nb) sm.hist[L] = 0;
+    wave_sync();
+    const int c = cur;
+    int placed = 0;  // fragments of earlier chunks
+    for (int base = 0; base < n; base += kWave) {
+      const int i = base + L;
+      int f = 0;
+      double plo = 0, phi = 0;
+      int pkey = 0;
+      bool live = false;
+      if (i < n) {
+        pkey = sm.key[c][i];
+        live = k - pkey <= rint;
+        plo = sm.lo[c][i];
+        phi = sm.hi[c][i];
+      }
+      if (live) {
+        if (plo < phi)
+          piece_fragments(plo, phi, nlo, nhi, nb, [&](double, double, int) { ++f; });
+        else
+          f = point_below(plo, nlo, nhi, nb) >= 0 ? 1 : 0;
+      }
+      int v = f;
+#pragma unroll
+      for (int d = 1; d < kWave; d <<= 1) {
+        const int o = __shfl_up(v, d);
+        if (L >= d) v += o;
+      }
+      const int total = placed + __shfl(v, kWave - 1);
+      if (total + nb > cap) {
+        raise_status(status, kStatusP1List);
+        return -1;
+      }
+      int q = placed + v - f;  // index of this piece's first fragment
+      if (f > 0) {
+        if (plo < phi) {
+          piece_fragments(plo, phi, nlo, nhi, nb, [&](double flo, double fhi, int below) {
+            const int pos = q + below;
+            sm.lo[c ^ 1][pos] = T(flo);
+            sm.hi[c ^ 1][pos] = T(fhi);
+            sm.key[c ^ 1][pos] = pkey;
+            atomicAdd(&sm.hist[below], 1);
+            ++q;
+          });
+        } else {
+          const int below = point_below(plo, nlo, nhi, nb);
+          sm.lo[c ^ 1][q + below] = T(plo);
+          sm.hi[c ^ 1][q + below] = T(phi);
+          sm.key[c ^ 1][q + below] = pkey;
+          atomicAdd(&sm.hist[below], 1);
+        }
+      }
+      placed = total;
+    }
+    wave_sync();
+    // batch segment j goes after the fragments with at most j segments below
+    int h = L <= nb ? sm.hist[L] : 0;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const int o = __shfl_up(h, d);
+      if (L >= d) h += o;
+    }
+    if (L < nb) {
+      sm.lo[c ^ 1][L + h] = nlo;
+      sm.hi[c ^ 1][L + h] = nhi;
+      sm.key[c ^ 1][L + h] = k;
+    }
+    n = placed + nb;
+    cur ^= 1;
+    wave_sync();
+  }
+  return n;
+}
+
+template <class T, int CAP, int ARENA, int CMAX>
+__global__ __launch_bounds__(kWave) void pass1_paint_kernel(Pass1Args<T> g) {
+  using S = Pass1PSmem<T, CAP, ARENA, CMAX>;
+  __shared__ S sm;
+  const int L = lane_id();
+  const int x = blockIdx.x % g.nx;
+  const int ci = blockIdx.x / g.nx;
+  const int y0 = ci * g.chunk;
+  const int y1 = min(g.ny, y0 + g.chunk);
+  if (y0 >= g.ny) return;
+  auto column = [&](int y) { return int64_t(x) + int64_t(g.nx) * y; };
+  // descriptor and first batch of a column, fetched one step ahead
+  auto fetch = [&](int y, int64_t& zo, int64_t& np, T& a, T& b) {
+    const int64_t col = column(y);
+    zo = g.in_off[col];
+    np = (g.in_off[col + 1] - zo) / 2;
+    a = b = T(0);
+    if (L < np) {
+      a = g.in_z[zo + 2 * L];
+      b = g.in_z[zo + 2 * L + 1];
+    }
+  };
+  // forward sweep, k = y
+  int cur = 0, n = 0, apos = 0;
+  const int ys = max(0, y0 - g.rint);
+  int64_t zo, np;
+  T pa, pb;
+  fetch(ys, zo, np, pa, pb);
+  for (int y = ys; y < y1; ++y) {
+    int64_t zo_n = 0, np_n = 0;
+    T pa_n = T(0), pb_n = T(0);
+    if (y + 1 < y1) fetch(y + 1, zo_n, np_n, pa_n, pb_n);
+    n = paint_step<T>(sm, cur, n, y, g.rint, g.in_z, zo, np, pa, pb, CAP, g.status);
+    if (n < 0) return;
+    if (y >= y0) {
+      if (apos + n > ARENA) {
+        raise_status(g.status, kStatusArenaOverflow);
+        return;
+      }
+      for (int i = L; i < n; i += kWave) {
+        sm.alo[apos + i] = sm.lo[cur][i];
+        sm.ahi[apos + i] = sm.hi[cur][i];
+        sm.ady[apos + i] = uint16_t(y - sm.key[cur][i]);
+      }
+      if (L == 0) {
+        sm.apos[y - y0] = apos;
+        sm.acnt[y - y0] = n;
+      }
+      apos += n;
+      wave_sync();
+    }
+    zo = zo_n;
+    np = np_n;
+    pa = pa_n;
+    pb = pb_n;
+  }
+  // backward sweep, k = ny-1-y; unite with the forward result and store
+  cur = 0;
+  n = 0;
+  const int ye = min(g.ny - 1, y1 - 1 + g.rint);
+  fetch(ye, zo, np, pa, pb);
+  for (int y = ye; y >= y0; --y) {
+    const int k = g.ny - 1 - y;
+    int64_t zo_n = 0, np_n = 0;
+    T pa_n = T(0), pb_n = T(0);
+    if (y - 1 >= y0) fetch(y - 1, zo_n, np_n, pa_n, pb_n);
+    n = paint_step<T>(sm, cur, n, k, g.rint, g.in_z, zo, np, pa, pb, CAP, g.status);
+    if (n < 0) return;
+    zo = zo_n;
+    np = np_n;
+    pa = pa_n;
+    pb = pb_n;
+    if (y >= y1) continue;
+    const int fa = sm.apos[y - y0], fn = sm.acnt[y - y0];
+    for (int i = L; i < fn; i += kWave) sm.ka[i] = -int(sm.ady[fa + i]);
+    for (int i = L; i < n; i += kWave) sm.kb[i] = -(k - sm.key[cur][i]);
+    wave_sync();
+    PieceRef<T> A{sm.alo + fa, sm.ahi + fa, sm.ka};
+    PieceRef<T> B{sm.lo[cur], sm.hi[cur], sm.kb};
+    PieceRef<T> U{sm.ulo, sm.uhi, sm.ukey};
+    MergeScratch<T> ms{sm.ev, sm.ec, sm.tlo, sm.thi, sm.tcode, S::ECAP};
+    const int nu = merge_max(A, fn, B, n, U, 2 * CAP, ms);
+    if (nu < 0 || nu > g.caps) {
+      raise_status(g.status, nu < 0 ? kStatusP1Union : kStatusArenaOverflow);
+      return;
+    }
+    for (int i = L; i < nu; i += kWave) {
+      const int64_t o = slot_index(g.nx, g.caps, x, y, i);
+      g.m_lo[o] = sm.ulo[i];
+      g.m_hi[o] = sm.uhi[i];
+      g.m_dy[o] = uint16_t(-sm.ukey[i]);
+    }
+    if (L == 0) g.m_cnt[column(y)] = nu;
+  }
+}
+
+}  // namespace vo
+
+// ---------------------------------------------------------------------------
+// Pass 1, z-sweep form.  Without culling, the pass-1 result at column (x, y)
+// is, for every z, the nearest line y' (|y - y'| <= R) whose input covers z,
+// which is what the union of the forward and backward half sweeps of
+// VoronoiVorPower.cpp:39-66 represents (both directions at once, nearer line
+// winning).  One wavefront per (x-line, 64 output columns): the segment
+// endpoints of the window of 64 + 2R input columns are counting-sorted by z
+// in LDS, then swept in z order with a coverage bit per window column; every
+// lane keeps the distance to its nearest covered column and emits a piece
+// whenever that distance changes.  Pieces come out z-sorted, already united
+// across both directions.  Windows with more endpoints than fit are swept in
+// z-slabs of consecutive histogram bins.
+// ---------------------------------------------------------------------------
+
+namespace vo {
+
+template <class T>
+struct Pass1ZArgs {
+  int nx, ny;
+  int rint;
+  const int64_t* in_off;
+  const T* in_z;
+  T* m_lo;  // slot layout, see Pass1Args
+  T* m_hi;
+  uint16_t* m_dy;
+  int caps;
+  int32_t* m_cnt;
+  int* status;
+};
+
+constexpr int kZTile = kWave;    // output columns per wavefront
+constexpr int kMaxSlabs = 64;    // z-slabs per window
+
+template <class T, int ECAP, int NB, int WMAX>
+struct Pass1ZSmem {
+  T ez[ECAP];
+  int ep[ECAP];  // (window column << 2) | kind: 0 = start, 1 = point, 2 = end
+  int bin[NB + 1];
+  int slab_end[kMaxSlabs + 1];
+};
+
+// Visits the merged segments of one input column (touching or overlapping
+// neighbours fused, points kept only when isolated).
+template <class T, class F>
+__device__ __forceinline__ void column_segments(const T* z, int64_t s, int64_t n, F&& f) {
+  if (n < 2) return;
+  T ca = z[s], cb = z[s + 1];
+  for (int64_t k = s + 2; k + 1 < s + n; k += 2) {
+    const T a = z[k], b = z[k + 1];
+    if (a <= cb) {
+      if (b > cb) cb = b;
+    } else {
+      f(ca, cb);
+      ca = a;
+      cb = b;
+    }
+  }
+  f(ca, cb);
+}
+
+template <int NW>
+__device__ __forceinline__ uint64_t pick_word(const uint64_t (&m)[NW], int i) {
+  uint64_t w = 0;
+#pragma unroll
+  for (int t = 0; t < NW; ++t) w = (i == t) ? m[t] : w;
+  return w;
+}
+
+// Distance from window position p to the nearest set bit within r, or r + 1.
+template <int NW>
+__device__ __forceinline__ int nearest_bit(const uint64_t (&m)[NW], int p, int r) {
+  int best = r + 1;
+  // at or above p
+  for (int q = p, span = 0; span <= r && q < NW * 64;) {
+    const int i = q >> 6, b = q & 63;
+    const uint64_t w = pick_word(m, i) >> b;
+    if (w) {
+      best = min(best, span + __builtin_ctzll(w));
+      break;
+    }
+    span += 64 - b;
+    q += 64 - b;
+  }
+  // at or below p
+  for (int q = p, span = 0; span <= r && q >= 0;) {
+    const int i = q >> 6, b = q & 63;
+    const uint64_t w = pick_word(m, i) << (63 - b);
+    if (w) {
+      best = min(best, span + __builtin_clzll(w));
+      break;
+    }
+    span += b + 1;
+    q -= b + 1;
+  }
+  return best;
+}
+
+template <class T, int ECAP, int NB, int WMAX>
+__global__ __launch_bounds__(kWave) void pass1_zsweep_kernel(Pass1ZArgs<T> g) {
+  using S = Pass1ZSmem<T, ECAP, NB, WMAX>;
+  constexpr int NW = (WMAX + 63) / 64;
+  __shared__ S sm;
+  const int L = lane_id();
+  const int x = blockIdx.x % g.nx;
+  const int y0 = (blockIdx.x / g.nx) * kZTile;
+  if (y0 >= g.ny) return;
+  const int R = g.rint;
+  const int wlo = y0 - R, wn = kZTile + 2 * R;
+  auto col_range = [&](int w, int64_t& s, int64_t& n) {
+    const int yy = wlo + w;
+    s = n = 0;
+    if (yy < 0 || yy >= g.ny) return;
+    const int64_t c = int64_t(x) + int64_t(g.nx) * yy;
+    s = g.in_off[c];
+    n = g.in_off[c + 1] - s;
+  };
+  // z-range of the window
+  double zlo = INFINITY, zhi = -INFINITY;
+  for (int w = L; w < wn; w += kWave) {
+    int64_t s, n;
+    col_range(w, s, n);
+    if (n >= 2) {
+      zlo = fmin(zlo, double(g.in_z[s]));
+      zhi = fmax(zhi, double(g.in_z[s + n - 1]));
+    }
+  }
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    zlo = fmin(zlo, __shfl_xor(zlo, d));
+    zhi = fmax(zhi, __shfl_xor(zhi, d));
+  }
+  const int64_t outcol0 = int64_t(x) + int64_t(g.nx) * (y0 + L);
+  const bool active = y0 + L < g.ny;
+  if (!(zlo <= zhi)) {  // no input anywhere in the window
+    if (active) g.m_cnt[outcol0] = 0;
+    return;
+  }
+  const double scale = double(NB) / fmax(zhi - zlo, 1e-30) * (1.0 - 1e-12);
+  auto bin_of = [&](T z) { return min(NB - 1, max(0, int((double(z) - zlo) * scale))); };
+  // histogram of the window's endpoints
+  for (int b = L; b <= NB; b += kWave) sm.bin[b] = 0;
+  wave_sync();
+  for (int w = L; w < wn; w += kWave) {
+    int64_t s, n;
+    col_range(w, s, n);
+    column_segments(g.in_z, s, n, [&](T a, T b) {
+      atomicAdd(&sm.bin[bin_of(a)], 1);
+      if (a < b) atomicAdd(&sm.bin[bin_of(b)], 1);
+    });
+  }
+  wave_sync();
+  // slabs of consecutive bins holding at most ECAP endpoints each: one slab
+  // when everything fits (the usual case), else a greedy split
+  int total = 0;
+  for (int b = L; b < NB; b += kWave) total += sm.bin[b];
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) total += __shfl_xor(total, d);
+  if (total <= ECAP) {
+    if (L == 0) {
+      sm.slab_end[0] = NB;
+      sm.slab_end[kMaxSlabs] = 1;
+    }
+  } else if (L == 0) {
+    int nslab = 0, acc = 0;
+    for (int b = 0; b < NB && nslab >= 0; ++b) {
+      const int c = sm.bin[b];
+      if (c > ECAP) {
+        nslab = -1;
+        break;
+      }
+      if (acc + c > ECAP) {
+        if (nslab + 1 >= kMaxSlabs) {
+          nslab = -1;
+          break;
+        }
+        sm.slab_end[nslab++] = b;
+        acc = 0;
+      }
+      acc += c;
+    }
+    if (nslab >= 0) sm.slab_end[nslab++] = NB;
+    sm.slab_end[kMaxSlabs] = nslab;
+  }
+  wave_sync();
+  const int nslab = sm.slab_end[kMaxSlabs];
+  if (nslab < 0) {
+    raise_status(g.status, kStatusP1List);
+    return;
+  }
+  // per-lane sweep state (lane = output column y0 + L, window position L + R)
+  const int p = L + R;
+  uint64_t mask[NW];
+#pragma unroll
+  for (int t = 0; t < NW; ++t) mask[t] = 0;
+  int dist = R + 1, dcur = R + 1, count = 0;
+  double z0 = 0;
+  bool point_open = false, ok = true;
+  // close the open piece and open the next one when the distance changed
+  auto emit = [&](double z, bool is_point) {
+    if (dist == dcur) return;
+    if (dcur <= R && (z0 < z || point_open) && active) {
+      if (count < g.caps) {
+        const int64_t o = slot_index(g.nx, g.caps, x, y0 + L, count);
+        g.m_lo[o] = T(z0);
+        g.m_hi[o] = T(z);
+        g.m_dy[o] = uint16_t(dcur);
+      } else {
+        ok = false;
+      }
+      ++count;
+    }
+    if (dist <= R) {
+      z0 = z;
+      point_open = is_point;
+    }
+    dcur = dist;
+  };
+  auto set_bit = [&](int w) {
+#pragma unroll
+    for (int t = 0; t < NW; ++t)
+      if ((w >> 6) == t) mask[t] |= uint64_t(1) << (w & 63);
+    const int dd = w > p ? w - p : p - w;
+    if (dd < dist) dist = dd;
+  };
+  auto clear_bit = [&](int w) {
+#pragma unroll
+    for (int t = 0; t < NW; ++t)
+      if ((w >> 6) == t) mask[t] &= ~(uint64_t(1) << (w & 63));
+    const int dd = w > p ? w - p : p - w;
+    if (dd == dist) dist = nearest_bit<NW>(mask, p, R);
+  };
+  int b_lo = 0;
+  for (int sl = 0; sl < nslab; ++sl) {
+    const int b_hi = sm.slab_end[sl];
+    wave_sync();
+    // exclusive bin starts of this slab
+    {
+      int carry = 0;
+      for (int base = b_lo; base < b_hi; base += kWave) {
+        const int b = base + L;
+        const int c = b < b_hi ? sm.bin[b] : 0;
+        int v = c;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+          const int o = __shfl_up(v, d);
+          if (L >= d) v += o;
+        }
+        if (b < b_hi) sm.bin[b] = carry + v - c;
+        carry += __shfl(v, kWave - 1);
+      }
+      wave_sync();
+      if (L == 0) sm.bin[NB] = carry;
+    }
+    wave_sync();
+    const int E = sm.bin[NB];
+    // scatter this slab's endpoints (kind: 0 start, 1 point, 2 end)
+    for (int w = L; w < wn; w += kWave) {
+      int64_t s, n;
+      col_range(w, s, n);
+      column_segments(g.in_z, s, n, [&](T a, T b) {
+        const int ba = bin_of(a);
+        if (!(a < b)) {
+          if (ba >= b_lo && ba < b_hi) {
+            const int q = atomicAdd(&sm.bin[ba], 1);
+            sm.ez[q] = a;
+            sm.ep[q] = (w << 2) | 1;
+          }
+          return;
+        }
+        if (ba >= b_lo && ba < b_hi) {
+          const int q = atomicAdd(&sm.bin[ba], 1);
+          sm.ez[q] = a;
+          sm.ep[q] = (w << 2) | 0;
+        }
+        const int bb = bin_of(b);
+        if (bb >= b_lo && bb < b_hi) {
+          const int q = atomicAdd(&sm.bin[bb], 1);
+          sm.ez[q] = b;
+          sm.ep[q] = (w << 2) | 2;
+        }
+      });
+    }
+    wave_sync();
+    // every bin now holds its end offset: order each bin by (z, kind)
+    for (int b = b_lo + L; b < b_hi; b += kWave) {
+      const int e0 = (b == b_lo) ? 0 : sm.bin[b - 1];
+      const int e1 = sm.bin[b];
+      for (int i = e0 + 1; i < e1; ++i) {
+        const T zi = sm.ez[i];
+        const int pi = sm.ep[i];
+        int j = i - 1;
+        while (j >= e0 && (sm.ez[j] > zi || (sm.ez[j] == zi && (sm.ep[j] & 3) > (pi & 3)))) {
+          sm.ez[j + 1] = sm.ez[j];
+          sm.ep[j + 1] = sm.ep[j];
+          --j;
+        }
+        sm.ez[j + 1] = zi;
+        sm.ep[j + 1] = pi;
+      }
+    }
+    wave_sync();
+    // sweep in (z, kind) order; a group of equal (z, kind) is applied whole
+    // before the lanes compare distances.  Events are read 64 at a time into
+    // registers and broadcast with readlane.
+    int gstart = 0;
+    for (int base = 0; base < E; base += kWave) {
+      const int cnt = min(kWave, E - base);
+      const T vz = L < cnt ? sm.ez[base + L] : T(0);
+      const int vp = L < cnt ? sm.ep[base + L] : 0;
+      const bool more = base + cnt < E;
+      const T zn_blk = more ? sm.ez[base + cnt] : T(0);
+      const int kn_blk = more ? (sm.ep[base + cnt] & 3) : -1;
+      for (int i = 0; i < cnt; ++i) {
+        const double z = double(lane_bcast(vz, i));
+        const int pk = __builtin_amdgcn_readlane(vp, i);
+        const int kind = pk & 3;
+        if (kind == 2)
+          clear_bit(pk >> 2);
+        else
+          set_bit(pk >> 2);
+        double zn = 0;
+        int kn = -1;
+        if (i + 1 < cnt) {
+          zn = double(lane_bcast(vz, i + 1));
+          kn = __builtin_amdgcn_readlane(vp, i + 1) & 3;
+        } else if (more) {
+          zn = double(zn_blk);
+          kn = kn_blk;
+        }
+        if (kn < 0 || zn != z || kn != kind) {
+          emit(z, kind == 1);
+          if (kind == 1) {
+            for (int t = gstart; t <= base + i; ++t) clear_bit(sm.ep[t] >> 2);
+            emit(z, false);
+          }
+          gstart = base + i + 1;
+        }
+      }
+    }
+    b_lo = b_hi;
+  }
+  if (ballot(active && !ok) != 0) raise_status(g.status, kStatusArenaOverflow);
+  if (active) g.m_cnt[outcol0] = count;
 }
 
 }  // namespace vo
