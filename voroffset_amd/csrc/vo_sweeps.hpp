@@ -907,30 +907,39 @@ __global__ __launch_bounds__(kWave) void pass2_gather_kernel(Pass2GArgs<T> g) {
   int n = 0;
   bool ok = true;
   double clo = INFINITY, chi = -INFINITY;  // an interval known to lie inside the union
-  constexpr int B = 8;  // source columns fetched together (independent loads in flight)
-  for (int j = 0; j < jmax; ++j) {
-    const int64_t srow = (int64_t(y) * g.caps + j) * g.nx;
-    for (int d0 = -rint; d0 <= rint; d0 += B) {
+  // source columns nearest first (d = 0, -1, +1, -2, ...): the widest
+  // dilations go in first, so later runs mostly fall inside the cached
+  // interval.  Within a source column the pieces are z-sorted and touching
+  // dilated pieces fuse into runs in registers; only run ends reach the list.
+  constexpr int B = 8;  // pieces fetched together (independent loads in flight)
+  for (int step = 0; step <= 2 * rint; ++step) {
+    const int ad = (step + 1) >> 1;
+    const int d = (step & 1) ? -ad : ad;
+    const int c = x + d;
+    const bool src = active && c >= 0 && c < g.nx;
+    const int m = src ? sm.cnt[L + d + rint] : 0;
+    int mmax = m;
+#pragma unroll
+    for (int t = 1; t < kWave; t <<= 1) mmax = max(mmax, __shfl_xor(mmax, t));
+    bool run = false;
+    double rs = 0, re = 0;
+    for (int j0 = 0; j0 < mmax; j0 += B) {
       T va[B], vb[B];
       int vdy[B];
-      bool live[B];
 #pragma unroll
       for (int u = 0; u < B; ++u) {
-        const int d = d0 + u, c = x + d;
-        live[u] = active && d <= rint && c >= 0 && c < g.nx && j < sm.cnt[L + d + rint];
         va[u] = vb[u] = T(0);
         vdy[u] = 0;
-        if (live[u]) {
-          va[u] = g.m_lo[srow + c];
-          vb[u] = g.m_hi[srow + c];
-          vdy[u] = g.m_dy[srow + c];
+        if (j0 + u < m) {
+          const int64_t o = (int64_t(y) * g.caps + j0 + u) * g.nx + c;
+          va[u] = g.m_lo[o];
+          vb[u] = g.m_hi[o];
+          vdy[u] = g.m_dy[o];
         }
       }
 #pragma unroll
       for (int u = 0; u < B; ++u) {
-        if (!live[u] || !ok) continue;
-        const int d = d0 + u;
-        const int ad = d < 0 ? -d : d;
+        if (j0 + u >= m || !ok) continue;
         const int dy = vdy[u];
         double h;
         if (table) {
@@ -941,12 +950,30 @@ __global__ __launch_bounds__(kWave) void pass2_gather_kernel(Pass2GArgs<T> g) {
         }
         if (h < 0) continue;
         const double s = double(va[u]) - h, e = double(vb[u]) + h;
-        if (s >= clo && e <= chi) continue;
-        int k = 0;
-        if (!lane_unite<OCAP>(sm.olo, sm.ohi, L, n, k, s, e)) {
-          ok = false;
+        if (run && s <= re) {  // pieces are z-sorted, so e >= rs: the run stays connected
+          rs = fmin(rs, s);
+          re = fmax(re, e);
           continue;
         }
+        if (run && !(rs >= clo && re <= chi)) {
+          int k = 0;
+          if (!lane_unite<OCAP>(sm.olo, sm.ohi, L, n, k, rs, re)) {
+            ok = false;
+            continue;
+          }
+          clo = sm.olo[k][L];
+          chi = sm.ohi[k][L];
+        }
+        run = true;
+        rs = s;
+        re = e;
+      }
+    }
+    if (ok && run && !(rs >= clo && re <= chi)) {
+      int k = 0;
+      if (!lane_unite<OCAP>(sm.olo, sm.ohi, L, n, k, rs, re))
+        ok = false;
+      else {
         clo = sm.olo[k][L];
         chi = sm.ohi[k][L];
       }
@@ -1299,47 +1326,99 @@ __device__ __forceinline__ void column_segments(const T* z, int64_t s, int64_t n
   f(ca, cb);
 }
 
-template <int NW>
-__device__ __forceinline__ uint64_t pick_word(const uint64_t (&m)[NW], int i) {
-  uint64_t w = 0;
-#pragma unroll
-  for (int t = 0; t < NW; ++t) w = (i == t) ? m[t] : w;
-  return w;
-}
+// Sweep state of one lane: coverage bits of the window columns (five 64-bit
+// words, held in registers), the distance to the nearest covered column and
+// the open piece.
+struct ZState {
+  uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0, m4 = 0;
+  int p = 0, R = 0;
+  int dist = 1 << 20, dcur = 1 << 20, count = 0;
+  double z0 = 0;
+  bool point_open = false, ok = true;
 
-// Distance from window position p to the nearest set bit within r, or r + 1.
-template <int NW>
-__device__ __forceinline__ int nearest_bit(const uint64_t (&m)[NW], int p, int r) {
-  int best = r + 1;
-  // at or above p
-  for (int q = p, span = 0; span <= r && q < NW * 64;) {
-    const int i = q >> 6, b = q & 63;
-    const uint64_t w = pick_word(m, i) >> b;
-    if (w) {
-      best = min(best, span + __builtin_ctzll(w));
-      break;
-    }
-    span += 64 - b;
-    q += 64 - b;
+  __device__ __forceinline__ uint64_t word(int i) const {
+    return i == 0 ? m0 : i == 1 ? m1 : i == 2 ? m2 : i == 3 ? m3 : i == 4 ? m4 : 0;
   }
-  // at or below p
-  for (int q = p, span = 0; span <= r && q >= 0;) {
-    const int i = q >> 6, b = q & 63;
-    const uint64_t w = pick_word(m, i) << (63 - b);
-    if (w) {
-      best = min(best, span + __builtin_clzll(w));
-      break;
+  __device__ __forceinline__ void flip(int w, bool on) {
+    const uint64_t b = uint64_t(1) << (w & 63);
+    const int i = w >> 6;
+    if (on) {
+      m0 |= i == 0 ? b : 0;
+      m1 |= i == 1 ? b : 0;
+      m2 |= i == 2 ? b : 0;
+      m3 |= i == 3 ? b : 0;
+      m4 |= i == 4 ? b : 0;
+    } else {
+      m0 &= i == 0 ? ~b : ~uint64_t(0);
+      m1 &= i == 1 ? ~b : ~uint64_t(0);
+      m2 &= i == 2 ? ~b : ~uint64_t(0);
+      m3 &= i == 3 ? ~b : ~uint64_t(0);
+      m4 &= i == 4 ? ~b : ~uint64_t(0);
     }
-    span += b + 1;
-    q -= b + 1;
   }
-  return best;
-}
+  // distance from p to the nearest set bit within R, or R + 1
+  __device__ __forceinline__ int nearest() const {
+    int best = R + 1;
+    for (int q = p, span = 0; span <= R && q < 5 * 64;) {
+      const int b = q & 63;
+      const uint64_t w = word(q >> 6) >> b;
+      if (w) {
+        best = min(best, span + __builtin_ctzll(w));
+        break;
+      }
+      span += 64 - b;
+      q += 64 - b;
+    }
+    for (int q = p, span = 0; span <= R && q >= 0;) {
+      const int b = q & 63;
+      const uint64_t w = word(q >> 6) << (63 - b);
+      if (w) {
+        best = min(best, span + __builtin_clzll(w));
+        break;
+      }
+      span += b + 1;
+      q -= b + 1;
+    }
+    return best;
+  }
+  __device__ __forceinline__ void set(int w) {
+    flip(w, true);
+    const int dd = w > p ? w - p : p - w;
+    if (dd < dist) dist = dd;
+  }
+  __device__ __forceinline__ void clear(int w) {
+    flip(w, false);
+    const int dd = w > p ? w - p : p - w;
+    if (dd == dist) dist = nearest();
+  }
+  // close the open piece and open the next one when the distance changed;
+  // zero-length pieces are kept only when a point opened them
+  template <class T>
+  __device__ __forceinline__ void emit(double z, bool is_point, bool active, const Pass1ZArgs<T>& g, int x, int y) {
+    if (dist == dcur) return;
+    if (dcur <= R && (z0 < z || point_open) && active) {
+      if (count < g.caps) {
+        const int64_t o = slot_index(g.nx, g.caps, x, y, count);
+        g.m_lo[o] = T(z0);
+        g.m_hi[o] = T(z);
+        g.m_dy[o] = uint16_t(dcur);
+      } else {
+        ok = false;
+      }
+      ++count;
+    }
+    if (dist <= R) {
+      z0 = z;
+      point_open = is_point;
+    }
+    dcur = dist;
+  }
+};
 
 template <class T, int ECAP, int NB, int WMAX>
 __global__ __launch_bounds__(kWave) void pass1_zsweep_kernel(Pass1ZArgs<T> g) {
   using S = Pass1ZSmem<T, ECAP, NB, WMAX>;
-  constexpr int NW = (WMAX + 63) / 64;
+  static_assert(WMAX <= 5 * 64, "the sweep state holds five mask words");
   __shared__ S sm;
   const int L = lane_id();
   const int x = blockIdx.x % g.nx;
@@ -1429,47 +1508,9 @@ __global__ __launch_bounds__(kWave) void pass1_zsweep_kernel(Pass1ZArgs<T> g) {
     return;
   }
   // per-lane sweep state (lane = output column y0 + L, window position L + R)
-  const int p = L + R;
-  uint64_t mask[NW];
-#pragma unroll
-  for (int t = 0; t < NW; ++t) mask[t] = 0;
-  int dist = R + 1, dcur = R + 1, count = 0;
-  double z0 = 0;
-  bool point_open = false, ok = true;
-  // close the open piece and open the next one when the distance changed
-  auto emit = [&](double z, bool is_point) {
-    if (dist == dcur) return;
-    if (dcur <= R && (z0 < z || point_open) && active) {
-      if (count < g.caps) {
-        const int64_t o = slot_index(g.nx, g.caps, x, y0 + L, count);
-        g.m_lo[o] = T(z0);
-        g.m_hi[o] = T(z);
-        g.m_dy[o] = uint16_t(dcur);
-      } else {
-        ok = false;
-      }
-      ++count;
-    }
-    if (dist <= R) {
-      z0 = z;
-      point_open = is_point;
-    }
-    dcur = dist;
-  };
-  auto set_bit = [&](int w) {
-#pragma unroll
-    for (int t = 0; t < NW; ++t)
-      if ((w >> 6) == t) mask[t] |= uint64_t(1) << (w & 63);
-    const int dd = w > p ? w - p : p - w;
-    if (dd < dist) dist = dd;
-  };
-  auto clear_bit = [&](int w) {
-#pragma unroll
-    for (int t = 0; t < NW; ++t)
-      if ((w >> 6) == t) mask[t] &= ~(uint64_t(1) << (w & 63));
-    const int dd = w > p ? w - p : p - w;
-    if (dd == dist) dist = nearest_bit<NW>(mask, p, R);
-  };
+  ZState st;
+  st.p = L + R;
+  st.R = R;
   int b_lo = 0;
   for (int sl = 0; sl < nslab; ++sl) {
     const int b_hi = sm.slab_end[sl];
@@ -1556,9 +1597,9 @@ __global__ __launch_bounds__(kWave) void pass1_zsweep_kernel(Pass1ZArgs<T> g) {
         const int pk = __builtin_amdgcn_readlane(vp, i);
         const int kind = pk & 3;
         if (kind == 2)
-          clear_bit(pk >> 2);
+          st.clear(pk >> 2);
         else
-          set_bit(pk >> 2);
+          st.set(pk >> 2);
         double zn = 0;
         int kn = -1;
         if (i + 1 < cnt) {
@@ -1569,10 +1610,10 @@ __global__ __launch_bounds__(kWave) void pass1_zsweep_kernel(Pass1ZArgs<T> g) {
           kn = kn_blk;
         }
         if (kn < 0 || zn != z || kn != kind) {
-          emit(z, kind == 1);
+          st.emit(z, kind == 1, active, g, x, y0 + L);
           if (kind == 1) {
-            for (int t = gstart; t <= base + i; ++t) clear_bit(sm.ep[t] >> 2);
-            emit(z, false);
+            for (int t = gstart; t <= base + i; ++t) st.clear(sm.ep[t] >> 2);
+            st.emit(z, false, active, g, x, y0 + L);
           }
           gstart = base + i + 1;
         }
@@ -1580,8 +1621,8 @@ __global__ __launch_bounds__(kWave) void pass1_zsweep_kernel(Pass1ZArgs<T> g) {
     }
     b_lo = b_hi;
   }
-  if (ballot(active && !ok) != 0) raise_status(g.status, kStatusArenaOverflow);
-  if (active) g.m_cnt[outcol0] = count;
+  if (ballot(active && !st.ok) != 0) raise_status(g.status, kStatusArenaOverflow);
+  if (active) g.m_cnt[outcol0] = st.count;
 }
 
 }  // namespace vo
