@@ -37,25 +37,39 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(vol, radius, threads, crop):
-    """Oracle `ours` on a crop x crop window of the workload; also returns the
-    full-volume mean pass-1 triple count m (needed for the byte figure)."""
-    import oracle
+def _crop(vol, crop):
     from voroffset_amd.volume import CompressedVolume
+    cols = [vol.at(x, y) for y in range(crop) for x in range(crop)]
+    return CompressedVolume.from_columns(crop, crop, cols, origin=vol.origin, extent=vol.extent,
+                                         spacing=vol.spacing, padding=vol.padding)
+
+
+def cpu_baseline(vol, radius, threads, crop, crop1):
+    """The oracle's `ours` (FP64 restatement, line-parallel with grain 10
+    lines like the reference's dormant TBB path) on crop x crop of the
+    workload with `threads` threads, and single-threaded on crop1 x crop1.
+    Also returns the full-volume mean pass-1 triple count m (for the byte
+    figure)."""
+    import oracle
 
     t = time.perf_counter()
     po = oracle.pass1(vol, radius, threads=threads)[0]
     m = float(np.diff(po).mean())
     log(f"oracle pass-1 count m={m:.3f} ({time.perf_counter() - t:.1f} s)")
-    cols = [vol.at(x, y) for y in range(crop) for x in range(crop)]
-    sample = CompressedVolume.from_columns(crop, crop, cols, origin=vol.origin, extent=vol.extent,
-                                           spacing=vol.spacing, padding=vol.padding)
+    sample = _crop(vol, crop)
     t = time.perf_counter()
     oracle.morph(sample, "dilation", radius, "ours", threads=threads)
     dt = time.perf_counter() - t
-    return m, {"value": sample.ncols / dt, "unit": "dexel columns/s", "cores": threads, "kind": "port",
+    single = _crop(vol, crop1)
+    t = time.perf_counter()
+    oracle.morph(single, "dilation", radius, "ours", threads=1)
+    dt1 = time.perf_counter() - t
+    return m, {"value": round(sample.ncols / dt, 1), "unit": "dexel columns/s", "cores": threads, "kind": "port",
                "sample": f"{crop}x{crop} crop of the same volume, oracle `ours` (FP64) dilation r={radius:g}, "
-                         f"{threads} threads, {dt:.1f} s"}
+                         f"{threads} threads, {dt:.1f} s",
+               "single_thread": {"value": round(single.ncols / dt1, 1), "cores": 1,
+                                 "sample": f"{crop1}x{crop1} crop, 1 thread, {dt1:.1f} s"},
+               "host_cpus": os.cpu_count()}
 
 
 def run_device(ctx, vol, radius, precision, steps, warmup):
@@ -92,7 +106,9 @@ def main():
     ap.add_argument("--radius", type=float, default=16.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--cpu-crop", type=int, default=384)
+    ap.add_argument("--cpu-crop", type=int, default=512, help="crop side of the all-threads CPU sample")
+    ap.add_argument("--cpu-crop1", type=int, default=192, help="crop side of the single-thread CPU sample")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 leg (counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -107,13 +123,19 @@ def main():
     ctx = _native.Context(0)
 
     step_s, p1_ms, p2_ms, count = run_device(ctx, vol, R, "fp32", args.steps, args.warmup)
-    log(f"fp32: {step_s * 1e3:.3f} ms/step, pass1 {p1_ms:.3f} ms, pass2 {p2_ms:.3f} ms")
-    f64_s, f64_p1, f64_p2, _ = run_device(ctx, vol, R, "fp64", max(3, args.steps // 4), 1)
-    log(f"fp64: {f64_s * 1e3:.3f} ms/step, pass1 {f64_p1:.3f} ms, pass2 {f64_p2:.3f} ms")
+    fallback = ctx.last_stats()[1]
+    log(f"fp32: {step_s * 1e3:.3f} ms/step, pass1 {p1_ms:.3f} ms, pass2 {p2_ms:.3f} ms, "
+        f"{fallback} pass-2 tiles recomputed by the sweep form")
+    plain = None
+    if not args.no_fp64:
+        f64_s, f64_p1, f64_p2, _ = run_device(ctx, vol, R, "fp64", max(3, args.steps // 4), 1)
+        log(f"fp64: {f64_s * 1e3:.3f} ms/step, pass1 {f64_p1:.3f} ms, pass2 {f64_p2:.3f} ms")
+        plain = {"value": round(ncols / f64_s, 1), "ms_per_step": round(f64_s * 1e3, 4),
+                 "pass1_ms": round(f64_p1, 4), "pass2_ms": round(f64_p2, 4)}
 
     m, cpu = (None, None)
     if not args.no_cpu:
-        m, cpu = cpu_baseline(vol, R, args.cpu_threads, args.cpu_crop)
+        m, cpu = cpu_baseline(vol, R, args.cpu_threads, args.cpu_crop, args.cpu_crop1)
         log(f"cpu baseline: {cpu['value']:.3g} columns/s")
 
     E = 4
@@ -151,7 +173,7 @@ def main():
                                f"Exp(40) gaps and lengths, dilation r={R:g}, one volume per launch",
                    "grid": [side, side], "radius": R, "storage": "fp32", "op": "dilation",
                    "pass1_ms": round(p1_ms, 4), "pass2_ms": round(p2_ms, 4),
-                   "plain_fp64": {"value": round(ncols / f64_s, 1), "ms_per_step": round(f64_s * 1e3, 4)},
+                   "plain_fp64": plain, "pass2_sweep_tiles": fallback,
                    "s_in": round(s_in, 4), "s_out": round(s_out, 4), "m_oracle": m},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -75,9 +75,25 @@ void vo_result_free(vo_result* r);
 int vo_morph_device(vo_context* ctx, int op, int precision, double radius, const vo_volume* in,
                     vo_volume* out, int64_t* out_count, double* time_1_ms, double* time_2_ms);
 
-/* Statistics of the last dilation on `ctx` (for the bandwidth figures):
- * number of pass-1 triples written and pass-2 output endpoints. */
-int vo_last_stats(const vo_context* ctx, int64_t* pass1_triples, int64_t* output_endpoints);
+/* Mesh -> dexels: replaces the ray casting of vor3d::create_dexels
+ * (src/vor3d/Dexelize.cpp:166-225; predicates :56-162).  `vertices` is
+ * num_vertices x 3 doubles (x, y, z), `triangles` num_triangles x 3 vertex
+ * indices.  Column x + nx*y of the host result holds the sorted z / spacing of
+ * every triangle whose xy-projection contains the dexel centre
+ * (origin_x + (x + 0.5) * spacing, origin_y + (y + 0.5) * spacing), with the
+ * reference's SOS tie-break on shared edges and vertices; consecutive pairs are
+ * the column's segments (even-odd).  The grid (nx, ny, origin, spacing) is the
+ * caller's, as create_dexels derives it from the mesh bounding box
+ * (Dexelize.cpp:252-270, CompressedVolume.cpp:11-23). */
+int vo_rasterize(vo_context* ctx, const double* vertices, int64_t num_vertices, const int32_t* triangles,
+                 int64_t num_triangles, int32_t nx, int32_t ny, double origin_x, double origin_y, double spacing,
+                 vo_result** out);
+
+/* Statistics of the last dilation on `ctx`: the number of output endpoints
+ * and the number of 64-column pass-2 tiles whose interval lists outgrew LDS
+ * and were recomputed by the sweep form (a tuning signal; results do not
+ * depend on it).  No reference counterpart. */
+int vo_last_stats(const vo_context* ctx, int64_t* output_endpoints, int64_t* fallback_tiles);
 
 #ifdef __cplusplus
 }

@@ -48,6 +48,10 @@ def lib():
             f = getattr(L, name)
             f.restype = ctypes.c_int
             f.argtypes = [_f64p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.vo_oracle_rasterize.restype = ctypes.c_void_p
+        L.vo_oracle_rasterize.argtypes = [_f64p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32), ctypes.c_int64,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_double]
         L.vo_oracle_union.restype = ctypes.c_int
         L.vo_oracle_union.argtypes = [_f64p, ctypes.c_int, _f64p, ctypes.c_int, _f64p, ctypes.c_int]
         _LIB = L
@@ -105,6 +109,26 @@ def pass1(vol, radius, threads=1):
     finally:
         L.vo_oracle_free(h)
     return o, y1, y2, r
+
+
+def rasterize(vertices, triangles, nx, ny, origin_x, origin_y, spacing):
+    """Mesh -> dexel columns (Dexelize.cpp:56-225): (off, z) with z / spacing
+    of the hits per column, sorted."""
+    L = lib()
+    V = np.ascontiguousarray(vertices, np.float64).reshape(-1, 3)
+    F = np.ascontiguousarray(triangles, np.int32).reshape(-1, 3)
+    h = L.vo_oracle_rasterize(_pf(V), V.shape[0], F.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), F.shape[0],
+                              int(nx), int(ny), float(origin_x), float(origin_y), float(spacing))
+    if not h:
+        raise RuntimeError(L.vo_oracle_last_error().decode())
+    try:
+        n = L.vo_oracle_count(h)
+        o = np.empty(int(nx) * int(ny) + 1, np.int64)
+        out = np.empty(n, np.float64)
+        L.vo_oracle_copy(h, _p64(o), _pf(out), None, None)
+    finally:
+        L.vo_oracle_free(h)
+    return o, out
 
 
 def negate_ray(col, lo, hi):

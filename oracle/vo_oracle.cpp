@@ -18,6 +18,7 @@
 //   Voronoi.cpp:8-89             erosion = crop o complement o dilation o complement
 //   VoronoiBruteForce.cpp:16-100 closed-form dilation
 //   app/cli3d/offset3d.cpp:124-133  closing / opening composition
+//   Dexelize.cpp:56-225          mesh -> dexels (ray casting, SOS predicates)
 //
 // How each part is pinned (the reference itself cannot be compiled here: its
 // core includes <Eigen/Dense>, which this image does not have):
@@ -33,6 +34,10 @@
 //     The exact triples themselves are "parity unpinned".
 //   * 1-D merges and complements: textbook interval union / complement on
 //     hand-made cases and random property checks.
+//   * rasterizer: closed forms (axis-aligned boxes, dexel centres exactly on
+//     shared edges and vertices, where SOS must count a point exactly once),
+//     and brute force over all triangles (the tile pruning only skips
+//     triangles whose bounding box misses the dexel centre).
 
 #include <algorithm>
 #include <atomic>
@@ -1009,6 +1014,116 @@ Result* to_result(const Volume& v) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Mesh -> dexels (Dexelize.cpp:56-225).  For each dexel centre
+// (origin_x + (x + 0.5) * spacing, origin_y + (y + 0.5) * spacing)
+// (CompressedVolumeBase.cpp:5-11), the z / spacing of every triangle whose
+// xy-projection contains it, sorted ascending (Dexelize.cpp:209).  Containment
+// is the SOS test of point_in_triangle_2d (Dexelize.cpp:56-92); a triangle
+// whose projection has zero inexact orientation is skipped (:107-121,151-158).
+// The reference prunes candidates with geogram's AABB tree (:207); pruning
+// never changes the result, so this restatement prunes with 8x8-dexel tiles.
+// Arithmetic follows the reference's operation order exactly (no FMA).
+
+namespace raster {
+
+int orientation(double x1, double y1, double x2, double y2, double& twice_area) {  // Dexelize.cpp:56-67
+  twice_area = y1 * x2 - x1 * y2;
+  if (twice_area > 0) return 1;
+  if (twice_area < 0) return -1;
+  if (y2 > y1) return 1;
+  if (y2 < y1) return -1;
+  if (x1 > x2) return 1;
+  if (x1 < x2) return -1;
+  return 0;
+}
+
+bool point_in_triangle(double x0, double y0, double x1, double y1, double x2, double y2, double x3,
+                       double y3, double& a, double& b, double& c) {  // Dexelize.cpp:73-92
+  x1 -= x0;
+  x2 -= x0;
+  x3 -= x0;
+  y1 -= y0;
+  y2 -= y0;
+  y3 -= y0;
+  const int sa = orientation(x2, y2, x3, y3, a);
+  if (sa == 0) return false;
+  if (orientation(x3, y3, x1, y1, b) != sa) return false;
+  if (orientation(x1, y1, x2, y2, c) != sa) return false;
+  const double sum = a + b + c;
+  a /= sum;
+  b /= sum;
+  c /= sum;
+  return true;
+}
+
+// intersect_ray_z (Dexelize.cpp:136-162): nonzero when the ray hits
+bool ray_z(const double* p1, const double* p2, const double* p3, double qx, double qy, double& z) {
+  double u, v, w;
+  if (!point_in_triangle(qx, qy, p1[0], p1[1], p2[0], p2[1], p3[0], p3[1], u, v, w)) return false;
+  z = u * p1[2] + v * p2[2] + w * p3[2];
+  const double a11 = p2[0] - p1[0], a12 = p2[1] - p1[1];  // orient_2d_inexact, :107-121
+  const double a21 = p3[0] - p1[0], a22 = p3[1] - p1[1];
+  const double delta = a11 * a22 - a12 * a21;
+  return delta != 0;
+}
+
+}  // namespace raster
+
+Result* rasterize(const double* V, int64_t nv, const int32_t* F, int64_t nf, int nx, int ny, double ox,
+                  double oy, double spacing) {
+  if (nx < 0 || ny < 0 || !(spacing > 0)) fail("rasterize: bad grid");
+  for (int64_t i = 0; i < 3 * nf; ++i)
+    if (F[i] < 0 || F[i] >= nv) fail("rasterize: triangle index out of range");
+  for (int64_t i = 0; i < 3 * nv; ++i)
+    if (!std::isfinite(V[i])) fail("rasterize: non-finite vertex");
+  constexpr int kT = 8;
+  const int tx = (nx + kT - 1) / kT, ty = (ny + kT - 1) / kT;
+  std::vector<std::vector<int64_t>> bins(size_t(tx) * ty);
+  for (int64_t f = 0; f < nf; ++f) {
+    double lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
+    for (int k = 0; k < 3; ++k)
+      for (int d = 0; d < 2; ++d) {
+        lo[d] = std::min(lo[d], V[3 * F[3 * f + k] + d]);
+        hi[d] = std::max(hi[d], V[3 * F[3 * f + k] + d]);
+      }
+    // dexel index range whose centres may lie in the closed bbox, widened by one
+    const double o[2] = {ox, oy};
+    int ilo[2], ihi[2];
+    const int n[2] = {nx, ny};
+    for (int d = 0; d < 2; ++d) {
+      const double a = std::floor((lo[d] - o[d]) / spacing - 0.5) - 1.0;
+      const double b = std::ceil((hi[d] - o[d]) / spacing - 0.5) + 1.0;
+      ilo[d] = int(std::min(std::max(a, 0.0), double(n[d])));
+      ihi[d] = int(std::max(std::min(b, double(n[d] - 1)), -1.0));
+    }
+    if (ilo[0] > ihi[0] || ilo[1] > ihi[1]) continue;
+    for (int b = ilo[1] / kT; b <= ihi[1] / kT; ++b)
+      for (int a = ilo[0] / kT; a <= ihi[0] / kT; ++a) bins[size_t(b) * tx + a].push_back(f);
+  }
+  auto* res = new Result;
+  res->nx = nx;
+  res->ny = ny;
+  std::vector<std::vector<double>> cols(size_t(nx) * ny);
+  for (int y = 0; y < ny; ++y)
+    for (int x = 0; x < nx; ++x) {
+      const double qx = (x + 0.5) * spacing + ox, qy = (y + 0.5) * spacing + oy;
+      auto& col = cols[size_t(x) + size_t(nx) * y];
+      for (int64_t f : bins[size_t(y / kT) * tx + x / kT]) {
+        double z;
+        if (raster::ray_z(V + 3 * F[3 * f], V + 3 * F[3 * f + 1], V + 3 * F[3 * f + 2], qx, qy, z))
+          col.push_back(z / spacing);
+      }
+      std::sort(col.begin(), col.end());
+    }
+  res->off.assign(cols.size() + 1, 0);
+  for (size_t c = 0; c < cols.size(); ++c) {
+    res->off[c + 1] = res->off[c] + int64_t(cols[c].size());
+    res->a.insert(res->a.end(), cols[c].begin(), cols[c].end());
+  }
+  return res;
+}
+
 extern "C" {
 
 const char* vo_oracle_last_error() { return g_error.c_str(); }
@@ -1046,6 +1161,18 @@ void* vo_oracle_pass1(int nx, int ny, const int64_t* off, const double* z, doubl
       }
     }
     return res;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return nullptr;
+  }
+}
+
+// Mesh -> dexels: V is nv x 3 (x, y, z), F is nf x 3 vertex indices; the
+// result holds, per column x + nx*y, the sorted z / spacing of the hits.
+void* vo_oracle_rasterize(const double* V, int64_t nv, const int32_t* F, int64_t nf, int nx, int ny,
+                          double ox, double oy, double spacing) {
+  try {
+    return rasterize(V, nv, F, nf, nx, ny, ox, oy, spacing);
   } catch (const std::exception& e) {
     g_error = e.what();
     return nullptr;

@@ -20,7 +20,7 @@ PRECISIONS = {"fp32": 0, "fp64": 1}
 
 # every symbol the header declares
 EXPORTS = ("vo_create", "vo_destroy", "vo_last_error", "vo_morph", "vo_result_shape", "vo_result_copy",
-           "vo_result_free", "vo_morph_device", "vo_last_stats")
+           "vo_result_free", "vo_morph_device", "vo_last_stats", "vo_rasterize")
 
 
 class VoError(RuntimeError):
@@ -65,6 +65,8 @@ def lib():
         L.vo_morph_device.restype = i32
         L.vo_last_stats.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
         L.vo_last_stats.restype = i32
+        L.vo_rasterize.argtypes = [vp, vp, i64, vp, i64, i32, i32, dbl, dbl, dbl, ctypes.POINTER(vp)]
+        L.vo_rasterize.restype = i32
         _lib = L
     return _lib
 
@@ -126,7 +128,31 @@ class Context:
             self._raise(rc)
         return vout, cnt.value, (t1.value, t2.value)
 
+    def rasterize(self, vertices, triangles, nx, ny, origin_x, origin_y, spacing):
+        """Mesh -> dexel columns (CSR offsets int64, z float64 in units of spacing)."""
+        V = np.ascontiguousarray(vertices, np.float64).reshape(-1, 3)
+        F = np.ascontiguousarray(triangles, np.int32).reshape(-1, 3)
+        res = ctypes.c_void_p()
+        rc = lib().vo_rasterize(self._h, V.ctypes.data if V.size else None, V.shape[0],
+                                F.ctypes.data if F.size else None, F.shape[0], int(nx), int(ny),
+                                float(origin_x), float(origin_y), float(spacing), ctypes.byref(res))
+        if rc != OK:
+            self._raise(rc)
+        return self._take(res)
+
+    def _take(self, res):
+        try:
+            rnx, rny, cnt = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
+            lib().vo_result_shape(res, ctypes.byref(rnx), ctypes.byref(rny), ctypes.byref(cnt))
+            o = np.empty(rnx.value * rny.value + 1, np.int64)
+            zz = np.empty(cnt.value, np.float64)
+            lib().vo_result_copy(res, o.ctypes.data, zz.ctypes.data if zz.size else None)
+        finally:
+            lib().vo_result_free(res)
+        return o, zz
+
     def last_stats(self):
+        """(output endpoints, pass-2 tiles recomputed by the sweep form) of the last dilation."""
         a, b = ctypes.c_int64(), ctypes.c_int64()
         lib().vo_last_stats(self._h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value

@@ -1,7 +1,6 @@
 """Builds the native library libvoroffset_hip.so in-tree for gfx950.
 
-    python -m voroffset_amd.build            # library only
-    python -m voroffset_amd.build --cli      # library + offset3d_hip CLI
+    python -m voroffset_amd.build            # library, vo_copy_probe, offset3d_hip CLI
 
 Every HIP translation unit is compiled with packed-FP32 instructions disabled
 (``-Xclang -target-feature -Xclang -packed-fp32-ops``).
@@ -46,15 +45,26 @@ def build_library(verbose=True):
     return LIB
 
 
+def build_copy_probe():
+    """The calibration copy kernel used for bandwidth and counter scaling."""
+    out = os.path.join(HERE, "vo_copy_probe")
+    run([hipcc(), *HIP_FLAGS, "-o", out + ".tmp", os.path.join(CSRC, "copy_probe.hip")])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_cli():
+    """offset3d_hip: host-only C++ (g++), linked against the library with an
+    $ORIGIN rpath so the pair can move together."""
     out = os.path.join(HERE, "offset3d_hip")
-    run([hipcc(), "-O2", "-std=c++17", "-o", out, os.path.join(CSRC, "offset3d.cpp"),
-         f"-I{os.path.join(ROOT, 'include')}", f"-L{HERE}", "-lvoroffset_hip",
-         f"-Wl,-rpath,{HERE}"])
+    gxx = shutil.which("g++") or "g++"
+    run([gxx, "-O2", "-std=c++17", "-Wall", "-o", out + ".tmp", os.path.join(CSRC, "offset3d.cpp"),
+         f"-I{os.path.join(ROOT, 'include')}", f"-L{HERE}", "-lvoroffset_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(out + ".tmp", out)
     return out
 
 
 if __name__ == "__main__":
     build_library()
-    if "--cli" in sys.argv:
-        build_cli()
+    build_copy_probe()
+    build_cli()

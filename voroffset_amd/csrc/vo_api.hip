@@ -19,11 +19,13 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <memory>
 #include <string>
 #include <type_traits>
 #include <vector>
 
 #include "../../include/voroffset_hip.h"
+#include "vo_raster.hpp"
 #include "vo_sweeps.hpp"
 
 namespace {
@@ -31,10 +33,12 @@ namespace {
 // Kernel shapes.  The LDS footprint of each instantiation is checked below.
 constexpr int kP1Cap = 128, kP1NCap = 64;
 constexpr int kP2SCap = 128, kP2PCap = 256, kP2NCap = 64, kP2OCap = 256;
-constexpr int kGOCap = 16;            // gather form: intervals per output column list
-constexpr int kGWMax = 64 + 2 * 128;  // gather form: widest source window (R < 129)
-constexpr int kGHT = 40;              // gather form: H table up to this radius
-constexpr int kZECap = 3072, kZNB = 1024, kZWMax = 64 + 2 * 128;  // pass-1 z-sweep form
+constexpr int kGHT = 40;  // gather form: H table in LDS up to this radius
+// gather form: intervals per output column list.  Large radii fuse the
+// dilated pieces into few intervals, so they get the short list (and more
+// waves per CU); a tile that overflows is recomputed by the sweep form.
+inline int gather_ocap(int rint) { return rint >= 6 ? 16 : 24; }
+constexpr int kZECap = 1536, kZNB = 512, kZWMax = 64 + 2 * 128;  // pass-1 z-sweep form
 
 template <class T>
 struct Shapes;
@@ -61,7 +65,7 @@ static_assert(sizeof(P1P<float>) <= 65536, "pass-1 painter LDS (fp32)");
 static_assert(sizeof(P1P<double>) <= 65536, "pass-1 painter LDS (fp64)");
 static_assert(sizeof(P2S<float>) <= 65536, "pass-2 LDS (fp32)");
 static_assert(sizeof(P2S<double>) <= 65536, "pass-2 LDS (fp64)");
-static_assert(sizeof(vo::Pass2GSmem<kGOCap, kGWMax, kGHT>) <= 65536, "pass-2 gather LDS");
+static_assert(vo::gather_lds_bytes(kGHT, 16, true) <= 65536, "pass-2 gather LDS");
 static_assert(sizeof(vo::Pass1ZSmem<double, kZECap, kZNB, kZWMax>) <= 65536, "pass-1 z-sweep LDS");
 
 struct Buffer {
@@ -75,6 +79,7 @@ enum Slot {
   kSTATUS, kSCAN_TMP, kTILES,
   kV0_OFF, kV0_Z, kV1_OFF, kV1_Z, kV2_OFF, kV2_Z, kV3_OFF, kV3_Z,  // device volumes
   kIN_OFF, kIN_Z,                                                   // staged host input
+  kR_V, kR_F, kR_TCNT, kR_TOFF, kR_TCUR, kR_TTRIS, kR_LEN, kR_OFF, kR_Z,  // rasterizer
   kSLOTS
 };
 
@@ -94,7 +99,7 @@ struct vo_context {
   Buffer buf[kSLOTS];
   int64_t* host_words = nullptr;  // pinned: status + counts
   std::string error;
-  int64_t last_triples = 0, last_outputs = 0;
+  int64_t last_outputs = 0;
   int p1_caps = 96;         // pass-1 slots per column (grows on overflow)
   int p2_cap_per_col = 16;  // pass-2 arena endpoints per column (grows on overflow)
   bool force_sweep = false;  // VO_PASS2=sweep: always use the sweep form of pass 2
@@ -210,28 +215,43 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
     T* m_lo = static_cast<T*>(ws(ctx, kM_LO, slots * sizeof(T)));
     T* m_hi = static_cast<T*>(ws(ctx, kM_HI, slots * sizeof(T)));
     uint16_t* m_dy = static_cast<uint16_t*>(ws(ctx, kM_DY, slots * sizeof(uint16_t)));
-    const int ntx = (nx + vo::kGatherTile - 1) / vo::kGatherTile;
-    const int64_t ntiles = int64_t(ny) * ntx;
-    const int64_t garena = int64_t(vo::kGatherTile) * 2 * kGOCap;  // endpoints per gather tile
-    const int64_t arena2 = int64_t(chunk2) * ctx->p2_cap_per_col;   // endpoints per sweep chunk
-    const int64_t fb_chunks = ntiles * (vo::kGatherTile / chunk2);  // worst case: every tile falls back
-    const int64_t o_total = ntiles * garena + fb_chunks * arena2;
+    // pass 2: gather tiles of (x, 64 consecutive y), and the sweep form,
+    // either over the whole volume in chunks of chunk2 columns of a row, or
+    // (list mode) over the gather tiles that overflowed, one column a block
+    const bool gather = rint <= 128 && !ctx->force_sweep;
+    const int64_t ntiles = int64_t(nx) * ((ny + vo::kGatherTile - 1) / vo::kGatherTile);
+    const int gocap = gather_ocap(rint);
+    const bool gtable = rint <= kGHT;
+    const int64_t garena = gather ? int64_t(vo::kGatherTile) * 2 * gocap : 0;  // endpoints per gather tile
+    const int chunk2_eff = gather ? 1 : chunk2;
+    const int64_t arena2 = int64_t(chunk2_eff) * ctx->p2_cap_per_col;  // endpoints per sweep block
+    const int64_t sweep_blocks = gather ? ntiles * vo::kGatherTile : int64_t(ny) * ((nx + chunk2 - 1) / chunk2);
+    const int64_t o_total = ntiles * garena + sweep_blocks * arena2;
     T* o_z = static_cast<T*>(ws(ctx, kO_Z, o_total * sizeof(T)));
     int* tiles = static_cast<int*>(ws(ctx, kTILES, (ntiles + 1) * sizeof(int)));
     check(ctx, hipMemsetAsync(status, 0, sizeof(int), ctx->stream), "memset status");
     check(ctx, hipMemsetAsync(tiles, 0, sizeof(int), ctx->stream), "memset tiles");
-    if (grid1 > INT32_MAX || fb_chunks > INT32_MAX) fail(ctx, VO_ERR_ARGUMENT, "grid too large");
+    if (grid1 > INT32_MAX || ntiles > INT32_MAX || sweep_blocks > INT32_MAX)
+      fail(ctx, VO_ERR_ARGUMENT, "grid too large");
 
     vo::Pass1Args<T> a1{nx, ny, rint, chunk1, nch1, in.off, static_cast<const T*>(in.z), m_lo, m_hi, m_dy,
                         caps, m_cnt, status};
     check(ctx, hipEventRecord(ctx->ev[0], ctx->stream), "event");
-    const bool zsweep = ctx->pass1_form == 0 && rint <= 128 && !zsweep_failed;
+    const bool zsweep = ctx->pass1_form == 0 && rint <= 127 && !zsweep_failed;
     if (zsweep) {
       vo::Pass1ZArgs<T> az{nx, ny, rint, in.off, static_cast<const T*>(in.z), m_lo, m_hi, m_dy, caps, m_cnt,
                            status};
       const int64_t gridz = int64_t(nx) * ((ny + vo::kZTile - 1) / vo::kZTile);
-      hipLaunchKernelGGL((vo::pass1_zsweep_kernel<T, kZECap, kZNB, kZWMax>), dim3(unsigned(gridz)),
-                         dim3(vo::kWave), 0, ctx->stream, az);
+      // coverage mask words per lane: 2R + 1 bits
+      if (rint <= 31)
+        hipLaunchKernelGGL((vo::pass1_zsweep_kernel<T, kZECap, kZNB, kZWMax, 1>), dim3(unsigned(gridz)),
+                           dim3(vo::kWave), 0, ctx->stream, az);
+      else if (rint <= 63)
+        hipLaunchKernelGGL((vo::pass1_zsweep_kernel<T, kZECap, kZNB, kZWMax, 2>), dim3(unsigned(gridz)),
+                           dim3(vo::kWave), 0, ctx->stream, az);
+      else
+        hipLaunchKernelGGL((vo::pass1_zsweep_kernel<T, kZECap, kZNB, kZWMax, 4>), dim3(unsigned(gridz)),
+                           dim3(vo::kWave), 0, ctx->stream, az);
     } else if (ctx->pass1_form == 2) {
       hipLaunchKernelGGL((vo::pass1_kernel<T, kP1Cap, kP1NCap, Shapes<T>::p1_arena, Shapes<T>::p1_cmax>),
                          dim3(unsigned(grid1)), dim3(vo::kWave), 0, ctx->stream, a1);
@@ -241,20 +261,20 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
     }
     check(ctx, hipGetLastError(), "pass1 launch");
     check(ctx, hipEventRecord(ctx->ev[1], ctx->stream), "event");
-    const bool gather = rint <= 128 && !ctx->force_sweep;
     int nfall = 0;
     if (gather) {
       vo::Pass2GArgs<T> ag{nx, ny, rint, R2, m_lo, m_hi, m_dy, caps, m_cnt, o_z, garena, o_start, o_len,
-                           tiles, status};
-      hipLaunchKernelGGL((vo::pass2_gather_kernel<T, kGOCap, kGWMax, kGHT>), dim3(unsigned(ntiles)),
-                         dim3(vo::kWave), 0, ctx->stream, ag);
+                           tiles, status, gocap, gtable ? 1 : 0};
+      const size_t lds = vo::gather_lds_bytes(rint, gocap, gtable);
+      hipLaunchKernelGGL((vo::pass2_gather_kernel<T>), dim3(unsigned(ntiles)), dim3(vo::kWave), lds,
+                         ctx->stream, ag);
       check(ctx, hipGetLastError(), "pass2 gather launch");
       nfall = int(read_word32(ctx, tiles));
     }
     // the sweep form: the whole volume, or the tiles the gather form gave back
-    const int64_t grid2 = gather ? int64_t(nfall) * (vo::kGatherTile / chunk2) : int64_t(ny) * ((nx + chunk2 - 1) / chunk2);
+    const int64_t grid2 = gather ? int64_t(nfall) * vo::kGatherTile : sweep_blocks;
     if (grid2 > 0) {
-      vo::Pass2Args<T> a2{nx, ny, rint, R2, chunk2, 0, m_lo, m_hi, m_dy, caps, m_cnt, o_z, arena2, o_start,
+      vo::Pass2Args<T> a2{nx, ny, rint, R2, chunk2_eff, 0, m_lo, m_hi, m_dy, caps, m_cnt, o_z, arena2, o_start,
                           o_len, status, gather ? tiles : nullptr, ntiles * garena};
       hipLaunchKernelGGL((vo::pass2_kernel<T, kP2SCap, kP2PCap, kP2NCap, kP2OCap, Shapes<T>::p2_farena,
                                            Shapes<T>::p2_cmax>),
@@ -410,6 +430,74 @@ void validate_host(vo_context* ctx, const vo_volume* v, int op, double radius) {
   }
 }
 
+// Mesh -> dexels (create_dexels' ray casting, Dexelize.cpp:166-225); see
+// vo_raster.hpp for the steps.
+void rasterize(vo_context* ctx, const double* V, int64_t nv, const int32_t* F, int64_t nf, int nx, int ny,
+               double ox, double oy, double spacing, vo_result** out) {
+  if (!out || nv < 0 || nf < 0 || nx < 0 || ny < 0 || (nv > 0 && !V) || (nf > 0 && !F))
+    fail(ctx, VO_ERR_ARGUMENT, "vo_rasterize: invalid arguments");
+  if (!(spacing > 0) || !std::isfinite(spacing) || !std::isfinite(ox) || !std::isfinite(oy))
+    fail(ctx, VO_ERR_ARGUMENT, "vo_rasterize: spacing must be finite and > 0, origin finite");
+  for (int64_t i = 0; i < 3 * nf; ++i)
+    if (F[i] < 0 || F[i] >= nv) fail(ctx, VO_ERR_ARGUMENT, "vo_rasterize: triangle index out of range");
+  for (int64_t i = 0; i < 3 * nv; ++i)
+    if (!std::isfinite(V[i])) fail(ctx, VO_ERR_ARGUMENT, "vo_rasterize: non-finite vertex");
+  const int64_t ncols = int64_t(nx) * ny;
+  std::unique_ptr<vo_result> res(new vo_result);
+  res->nx = nx;
+  res->ny = ny;
+  res->off.assign(size_t(ncols) + 1, 0);
+  if (ncols == 0 || nf == 0) {
+    *out = res.release();
+    return;
+  }
+  vo::RasterGrid g{nx, ny, ox, oy, spacing, (nx + vo::kRasterTile - 1) / vo::kRasterTile,
+                   (ny + vo::kRasterTile - 1) / vo::kRasterTile};
+  const int64_t ntiles = int64_t(g.tx) * g.ty;
+  if (ntiles > INT32_MAX || nf > int64_t(INT32_MAX) * 256) fail(ctx, VO_ERR_ARGUMENT, "vo_rasterize: too large");
+  double* dV = static_cast<double*>(ws(ctx, kR_V, size_t(nv) * 3 * sizeof(double)));
+  int32_t* dF = static_cast<int32_t*>(ws(ctx, kR_F, size_t(nf) * 3 * sizeof(int32_t)));
+  int64_t* tcnt = static_cast<int64_t*>(ws(ctx, kR_TCNT, size_t(ntiles + 1) * sizeof(int64_t)));
+  int64_t* toff = static_cast<int64_t*>(ws(ctx, kR_TOFF, size_t(ntiles + 1) * sizeof(int64_t)));
+  int64_t* tcur = static_cast<int64_t*>(ws(ctx, kR_TCUR, size_t(ntiles + 1) * sizeof(int64_t)));
+  int64_t* len = static_cast<int64_t*>(ws(ctx, kR_LEN, size_t(ncols + 1) * sizeof(int64_t)));
+  int64_t* off = static_cast<int64_t*>(ws(ctx, kR_OFF, size_t(ncols + 1) * sizeof(int64_t)));
+  check(ctx, hipMemcpyAsync(dV, V, size_t(nv) * 3 * sizeof(double), hipMemcpyHostToDevice, ctx->stream), "upload");
+  check(ctx, hipMemcpyAsync(dF, F, size_t(nf) * 3 * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream), "upload");
+  check(ctx, hipMemsetAsync(tcnt, 0, size_t(ntiles + 1) * sizeof(int64_t), ctx->stream), "memset");
+  const dim3 tgrid(blocks_for(nf, 256));
+  hipLaunchKernelGGL(vo::raster_bin_kernel, tgrid, dim3(256), 0, ctx->stream, dV, dF, nf, g, 0, tcnt, nullptr,
+                     nullptr);
+  check(ctx, hipGetLastError(), "raster bin launch");
+  exclusive_scan(ctx, tcnt, toff, ntiles + 1);
+  const int64_t pairs = read_word(ctx, toff + ntiles);
+  int64_t* ttris = static_cast<int64_t*>(ws(ctx, kR_TTRIS, size_t(std::max<int64_t>(pairs, 1)) * sizeof(int64_t)));
+  check(ctx, hipMemcpyAsync(tcur, toff, size_t(ntiles + 1) * sizeof(int64_t), hipMemcpyDeviceToDevice, ctx->stream),
+        "copy");
+  hipLaunchKernelGGL(vo::raster_bin_kernel, tgrid, dim3(256), 0, ctx->stream, dV, dF, nf, g, 1, nullptr, tcur,
+                     ttris);
+  check(ctx, hipGetLastError(), "raster bin launch");
+  hipLaunchKernelGGL(vo::raster_cast_kernel, dim3(unsigned(ntiles)), dim3(vo::kWave), 0, ctx->stream, dV, dF, g,
+                     toff, ttris, 0, len, nullptr, nullptr);
+  check(ctx, hipGetLastError(), "raster cast launch");
+  check(ctx, hipMemsetAsync(len + ncols, 0, sizeof(int64_t), ctx->stream), "memset");
+  exclusive_scan(ctx, len, off, ncols + 1);
+  const int64_t hits = read_word(ctx, off + ncols);
+  double* dz = static_cast<double*>(ws(ctx, kR_Z, size_t(std::max<int64_t>(hits, 1)) * sizeof(double)));
+  hipLaunchKernelGGL(vo::raster_cast_kernel, dim3(unsigned(ntiles)), dim3(vo::kWave), 0, ctx->stream, dV, dF, g,
+                     toff, ttris, 1, nullptr, off, dz);
+  check(ctx, hipGetLastError(), "raster cast launch");
+  res->z.resize(size_t(hits));
+  check(ctx, hipMemcpyAsync(res->off.data(), off, size_t(ncols + 1) * sizeof(int64_t), hipMemcpyDeviceToHost,
+                            ctx->stream),
+        "download");
+  if (hits)
+    check(ctx, hipMemcpyAsync(res->z.data(), dz, size_t(hits) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+          "download");
+  check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+  *out = res.release();
+}
+
 template <class T>
 void morph_host(vo_context* ctx, int op, double R, const vo_volume* in, vo_result** out, double* t1,
                 double* t2) {
@@ -527,6 +615,21 @@ int vo_result_copy(const vo_result* r, int64_t* offsets, double* z) {
 
 void vo_result_free(vo_result* r) { delete r; }
 
+int vo_rasterize(vo_context* ctx, const double* vertices, int64_t num_vertices, const int32_t* triangles,
+                 int64_t num_triangles, int32_t nx, int32_t ny, double origin_x, double origin_y, double spacing,
+                 vo_result** out) {
+  vo_result* r = nullptr;
+  const int rc = guarded(ctx, [&] {
+    rasterize(ctx, vertices, num_vertices, triangles, num_triangles, nx, ny, origin_x, origin_y, spacing, &r);
+  });
+  if (rc != VO_OK) {
+    delete r;
+    return rc;
+  }
+  *out = r;
+  return VO_OK;
+}
+
 int vo_morph_device(vo_context* ctx, int op, int precision, double radius, const vo_volume* in, vo_volume* out,
                     int64_t* out_count, double* time_1_ms, double* time_2_ms) {
   return guarded(ctx, [&] {
@@ -559,10 +662,10 @@ int vo_morph_device(vo_context* ctx, int op, int precision, double radius, const
   });
 }
 
-int vo_last_stats(const vo_context* ctx, int64_t* pass1_triples, int64_t* output_endpoints) {
+int vo_last_stats(const vo_context* ctx, int64_t* output_endpoints, int64_t* fallback_tiles) {
   if (!ctx) return VO_ERR_ARGUMENT;
-  if (pass1_triples) *pass1_triples = ctx->last_triples;
   if (output_endpoints) *output_endpoints = ctx->last_outputs;
+  if (fallback_tiles) *fallback_tiles = ctx->last_fallback_tiles;
   return VO_OK;
 }
 

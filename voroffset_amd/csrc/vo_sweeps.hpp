@@ -46,18 +46,24 @@ struct Pass1Args {
   const int64_t* in_off;
   const T* in_z;
   // output triples in slot layout: triple j of column (x, y) at
-  // (y * caps + j) * nx + x, so that a row's j-th triples are contiguous
+  // (x * caps + j) * ny + y, so that the j-th triples of an x-line are
+  // contiguous (see slot_index)
   T* m_lo;
   T* m_hi;
   uint16_t* m_dy;
   int caps;        // slots per column
-  int32_t* m_cnt;  // per column: number of triples
+  int32_t* m_cnt;  // per column, at count_index: number of triples
   int* status;
 };
 
-__device__ __forceinline__ int64_t slot_index(int nx, int caps, int x, int y, int j) {
-  return (int64_t(y) * caps + j) * nx + x;
+// Slot layout of the pass-1 triples, y-minor: the j-th triples of the
+// columns (x, y .. y + 63) share cache lines.  Pass 1 computes 64 consecutive
+// y of one x-line per wavefront, so each wavefront fills whole lines itself;
+// pass 2 reads them back with the same lane-to-y mapping.
+__device__ __forceinline__ int64_t slot_index(int ny, int caps, int x, int y, int j) {
+  return (int64_t(x) * caps + j) * ny + y;
 }
+__device__ __forceinline__ int64_t count_index(int ny, int x, int y) { return int64_t(x) * ny + y; }
 
 template <class T, int CAP, int NCAP, int ARENA, int CMAX>
 struct Pass1Smem {
@@ -231,12 +237,12 @@ __global__ __launch_bounds__(kWave) void pass1_kernel(Pass1Args<T> g) {
       return;
     }
     for (int i = L; i < nu; i += kWave) {
-      const int64_t o = slot_index(g.nx, g.caps, x, y, i);
+      const int64_t o = slot_index(g.ny, g.caps, x, y, i);
       g.m_lo[o] = sm.ulo[i];
       g.m_hi[o] = sm.uhi[i];
       g.m_dy[o] = uint16_t(-sm.ukey[i]);
     }
-    if (L == 0) g.m_cnt[col] = nu;
+    if (L == 0) g.m_cnt[count_index(g.ny, x, y)] = nu;
   }
 }
 
@@ -262,7 +268,7 @@ struct Pass2Args {
   int64_t* o_cnt;    // per column, in endpoints
   int* status;
   // list mode: recompute only the gather tiles listed in tiles[1 .. tiles[0]]
-  // (kGatherTile columns each, split into chunks of `chunk` columns), with
+  // (columns (x, y0 .. y0 + 63), one block of chunk 1 per column), with
   // arenas from o_base on
   const int* tiles;
   int64_t o_base;
@@ -391,7 +397,7 @@ __device__ int pass2_step(S& sm, int& cs, int& ns, int& cp, int& np, int k, int 
   for (int b0 = 0; b0 < mcnt; b0 += ncap) {
     const int nb = min(ncap, mcnt - b0);
     for (int i = L; i < nb; i += kWave) {
-      const int64_t o = slot_index(g.nx, g.caps, xcol, ycol, b0 + i);
+      const int64_t o = slot_index(g.ny, g.caps, xcol, ycol, b0 + i);
       const int dy = g.m_dy[o];
       sm.nlo[i] = g.m_lo[o];
       sm.nhi[i] = g.m_hi[o];
@@ -634,11 +640,11 @@ __global__ __launch_bounds__(kWave) void pass2_kernel(Pass2Args<T> g) {
   __shared__ S sm;
   const int L = lane_id();
   int y, x0;
-  if (g.tiles) {
-    const int per = kGatherTile / g.chunk;
-    const int tile = g.tiles[1 + blockIdx.x / per];
-    y = tile % g.ny;
-    x0 = (tile / g.ny) * kGatherTile + (blockIdx.x % per) * g.chunk;
+  if (g.tiles) {  // gather tile (x, y0 .. y0 + 63): one block per y, chunk 1
+    const int tile = g.tiles[1 + blockIdx.x / kGatherTile];
+    x0 = tile % g.nx;
+    y = (tile / g.nx) * kGatherTile + blockIdx.x % kGatherTile;
+    if (y >= g.ny) return;
   } else {
     y = blockIdx.x % g.ny;
     x0 = (blockIdx.x / g.ny) * g.chunk;
@@ -650,8 +656,7 @@ __global__ __launch_bounds__(kWave) void pass2_kernel(Pass2Args<T> g) {
   // forward sweep, k = x
   int cs = 0, ns = 0, cp = 0, np = 0, fpos = 0;
   for (int x = max(0, x0 - g.rint); x < x1; ++x) {
-    const int64_t col = int64_t(x) + int64_t(g.nx) * y;
-    const int nu = pass2_step<T>(sm, cs, ns, cp, np, x, x, y, g.m_cnt[col], g, NCAP, SCAP, PCAP, OCAP,
+    const int nu = pass2_step<T>(sm, cs, ns, cp, np, x, x, y, g.m_cnt[count_index(g.ny, x, y)], g, NCAP, SCAP, PCAP, OCAP,
                                  sm.ulo[2], sm.uhi[2]);
     if (nu < 0) return;
     if (x >= x0) {
@@ -680,7 +685,7 @@ __global__ __launch_bounds__(kWave) void pass2_kernel(Pass2Args<T> g) {
   for (int x = min(g.nx - 1, x1 - 1 + g.rint); x >= x0; --x) {
     const int k = g.nx - 1 - x;
     const int64_t col = int64_t(x) + int64_t(g.nx) * y;
-    const int nu = pass2_step<T>(sm, cs, ns, cp, np, k, x, y, g.m_cnt[col], g, NCAP, SCAP, PCAP, OCAP,
+    const int nu = pass2_step<T>(sm, cs, ns, cp, np, k, x, y, g.m_cnt[count_index(g.ny, x, y)], g, NCAP, SCAP, PCAP, OCAP,
                                  sm.ulo[2], sm.uhi[2]);
     if (nu < 0) return;
     if (x >= x1) continue;
@@ -797,12 +802,13 @@ __global__ void crop_complement_kernel(int nx, int ny, const int64_t* off, const
 // columns x' with |x - x'| = d <= R of the pass-1 pieces dilated by
 // H = sqrt(R^2 - dy^2 - d^2): exactly what SeparatePower2D's forward and
 // backward sweeps assemble (their getLine emits [y1 - dz, y2 + dz] for every
-// live interior and point, SeparatePower2D.cpp:295-341).  One lane per output
-// column.  Piece j of the 2R+1 source columns of every lane is read with one
-// coalesced load per d (the slot layout keeps a row's j-th pieces
-// contiguous), dilated with H from an LDS table, and united into the lane's
-// sorted output list in LDS; a cached covering interval skips the pieces that
-// change nothing.  A tile whose lists overflow is recomputed by the sweep form.
+// live interior and point, SeparatePower2D.cpp:295-341).  One wavefront per
+// tile of 64 output columns (x, y0 .. y0 + 63), one lane per column.  Piece j
+// of source column (x + d, y) is read with one coalesced load per d for the
+// whole wavefront (the slot layout is y-minor), dilated with H from an LDS
+// table, and united into the lane's sorted output list in LDS, whose current
+// interval is cached in registers.  A tile whose lists overflow is recomputed
+// by the sweep form, column by column.
 // ---------------------------------------------------------------------------
 
 namespace vo {
@@ -823,127 +829,157 @@ struct Pass2GArgs {
   int64_t* o_cnt;
   int* tile_list;  // [0] = count, then the ids of tiles to recompute with the sweep form
   int* status;
+  int ocap;        // intervals per output column list
+  int table;       // nonzero: H comes from an LDS table of (rint + 1)^2 entries
 };
 
-template <int OCAP, int WMAX, int HT>
-struct Pass2GSmem {
-  int cnt[WMAX];
-  double h[(HT + 1) * (HT + 1)];
-  double olo[OCAP][kWave];
-  double ohi[OCAP][kWave];
-};
+// Dynamic LDS of the gather form, in this order: the interval lists (lo, hi;
+// [ocap][64] doubles each, lane-minor so a wavefront's accesses are
+// conflict-free) and the H table.
+__host__ __device__ constexpr size_t gather_lds_bytes(int rint, int ocap, bool table) {
+  return sizeof(double) * (2 * size_t(ocap) * kWave + (table ? size_t(rint + 1) * (rint + 1) : 0));
+}
 
 // Unites [s, e] into the lane's sorted disjoint list (touching fuses),
 // starting the search at cursor k; on return k indexes the interval that now
 // holds [s, e].  Returns false on overflow.
-template <int OCAP>
-__device__ __forceinline__ bool lane_unite(double (*olo)[kWave], double (*ohi)[kWave], int L, int& n, int& k,
-                                           double s, double e) {
-  while (k > 0 && ohi[k - 1][L] >= s) --k;
-  while (k < n && ohi[k][L] < s) ++k;
-  if (k == n || olo[k][L] > e) {
-    if (n == OCAP) return false;
+// The lists are lane-minor: entry k of this lane is at [k * 64] of the
+// lane's base pointers olo, ohi.
+__device__ __forceinline__ bool lane_unite(double* olo, double* ohi, int ocap, int& n, int& k, double s,
+                                           double e) {
+  while (k > 0 && ohi[(k - 1) * kWave] >= s) --k;
+  while (k < n && ohi[k * kWave] < s) ++k;
+  if (k == n || olo[k * kWave] > e) {
+    if (n == ocap) return false;
     for (int j = n; j > k; --j) {
-      olo[j][L] = olo[j - 1][L];
-      ohi[j][L] = ohi[j - 1][L];
+      olo[j * kWave] = olo[(j - 1) * kWave];
+      ohi[j * kWave] = ohi[(j - 1) * kWave];
     }
-    olo[k][L] = s;
-    ohi[k][L] = e;
+    olo[k * kWave] = s;
+    ohi[k * kWave] = e;
     ++n;
     return true;
   }
-  if (s < olo[k][L]) olo[k][L] = s;
-  double hi = fmax(ohi[k][L], e);
+  if (s < olo[k * kWave]) olo[k * kWave] = s;
+  double hi = fmax(ohi[k * kWave], e);
   int j = k + 1;
-  while (j < n && olo[j][L] <= hi) {
-    hi = fmax(hi, ohi[j][L]);
+  while (j < n && olo[j * kWave] <= hi) {
+    hi = fmax(hi, ohi[j * kWave]);
     ++j;
   }
   const int gone = j - k - 1;
   if (gone > 0) {
     for (int t = k + 1; t + gone < n; ++t) {
-      olo[t][L] = olo[t + gone][L];
-      ohi[t][L] = ohi[t + gone][L];
+      olo[t * kWave] = olo[(t + gone) * kWave];
+      ohi[t * kWave] = ohi[(t + gone) * kWave];
     }
     n -= gone;
   }
-  ohi[k][L] = hi;
+  ohi[k * kWave] = hi;
   return true;
 }
 
-template <class T, int OCAP, int WMAX, int HT>
+extern __shared__ double vo_gather_lds[];
+
+template <class T>
 __global__ __launch_bounds__(kWave) void pass2_gather_kernel(Pass2GArgs<T> g) {
-  using S = Pass2GSmem<OCAP, WMAX, HT>;
-  __shared__ S sm;
   const int L = lane_id();
-  const int y = blockIdx.x % g.ny;
-  const int x0 = (blockIdx.x / g.ny) * kGatherTile;
-  if (x0 >= g.nx) return;
+  const int ocap = g.ocap;
+  double* olo = vo_gather_lds + L;
+  double* ohi = vo_gather_lds + ocap * kWave + L;
+  double* htab = vo_gather_lds + 2 * ocap * kWave;
+  // tile = (x, y0 .. y0 + 63); consecutive tiles are consecutive x, whose
+  // source windows overlap, so they meet in one XCD's L2
+  const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int x = int(tile % g.nx);
+  const int y0 = int(tile / g.nx) * kGatherTile;
+  if (y0 >= g.ny) return;
   if (*g.status != 0) return;  // pass 1 did not complete
   const int rint = g.rint;
-  const int wlo = x0 - rint, wn = kGatherTile + 2 * rint;
-  const int64_t rowbase = int64_t(g.nx) * y;
+  const int y = y0 + L;
+  const bool active = y < g.ny;
+  const int c_lo = max(0, x - rint), c_hi = min(g.nx - 1, x + rint);
+  auto count_of = [&](int c) { return active ? min(g.m_cnt[count_index(g.ny, c, y)], g.caps) : 0; };
   int jmax = 0;
-  for (int w = L; w < wn; w += kWave) {
-    const int c = wlo + w;
-    const int m = (c >= 0 && c < g.nx) ? min(g.m_cnt[rowbase + c], g.caps) : 0;
-    sm.cnt[w] = m;
-    jmax = max(jmax, m);
-  }
+  for (int c = c_lo; c <= c_hi; ++c) jmax = max(jmax, count_of(c));
 #pragma unroll
   for (int d = 1; d < kWave; d <<= 1) jmax = max(jmax, __shfl_xor(jmax, d));
-  const bool table = rint <= HT;
+  const bool table = g.table != 0;
   if (table) {
     const int w1 = rint + 1;
     for (int i = L; i < w1 * w1; i += kWave) {
       const int dy = i / w1, ad = i % w1;
       const double v = g.R2 - double(dy) * double(dy) - double(ad) * double(ad);
-      sm.h[i] = v >= 0 ? sqrt(v) : -1.0;
+      htab[i] = v >= 0 ? sqrt(v) : -1.0;
     }
   }
   wave_sync();
-  const int x = x0 + L;
-  const bool active = x < g.nx;
   int n = 0;
   bool ok = true;
-  double clo = INFINITY, chi = -INFINITY;  // an interval known to lie inside the union
-  // source columns nearest first (d = 0, -1, +1, -2, ...): the widest
-  // dilations go in first, so later runs mostly fall inside the cached
-  // interval.  Within a source column the pieces are z-sorted and touching
-  // dilated pieces fuse into runs in registers; only run ends reach the list.
-  constexpr int B = 8;  // pieces fetched together (independent loads in flight)
-  for (int step = 0; step <= 2 * rint; ++step) {
-    const int ad = (step + 1) >> 1;
-    const int d = (step & 1) ? -ad : ad;
-    const int c = x + d;
-    const bool src = active && c >= 0 && c < g.nx;
-    const int m = src ? sm.cnt[L + d + rint] : 0;
-    int mmax = m;
-#pragma unroll
-    for (int t = 1; t < kWave; t <<= 1) mmax = max(mmax, __shfl_xor(mmax, t));
-    bool run = false;
-    double rs = 0, re = 0;
-    for (int j0 = 0; j0 < mmax; j0 += B) {
+  // Interval k of the list is cached in registers as [clo, chi], with the
+  // bounds of its neighbours (below = hi of k - 1, above = lo of k + 1): a run
+  // that meets it without reaching a neighbour is united in registers, and
+  // only the other runs go through the LDS list.
+  int k = 0;
+  double clo = INFINITY, chi = -INFINITY, below = -INFINITY, above = INFINITY;
+  bool dirty = false;
+  auto unite = [&](double s, double e) {
+    if (s <= chi && e >= clo && s > below && e < above) {
+      clo = fmin(clo, s);
+      chi = fmax(chi, e);
+      dirty = true;
+      return;
+    }
+    if (dirty) {
+      olo[k * kWave] = clo;
+      ohi[k * kWave] = chi;
+      dirty = false;
+    }
+    if (!lane_unite(olo, ohi, ocap, n, k, s, e)) {
+      ok = false;
+      return;
+    }
+    clo = olo[k * kWave];
+    chi = ohi[k * kWave];
+    below = k > 0 ? ohi[(k - 1) * kWave] : -INFINITY;
+    above = k + 1 < n ? olo[(k + 1) * kWave] : INFINITY;
+  };
+  // Blocks of B pieces (j) outermost, source columns nearest first within a
+  // block (d = 0, -1, +1, -2, ...): a block touches a few cache lines per
+  // source column, which the neighbouring tiles read at about the same time.
+  // Within a source column the pieces are z-sorted, and touching dilated
+  // pieces fuse into a run in registers before they reach the list.
+  constexpr int B = 8;
+  for (int jb = 0; jb < jmax; jb += B) {
+    for (int step = 0; step <= 2 * rint; ++step) {
+      const int ad = (step + 1) >> 1;
+      const int d = (step & 1) ? -ad : ad;
+      const int c = x + d;
+      if (c < 0 || c >= g.nx) continue;  // uniform
+      const int m = count_of(c);
+      if (ballot(m > jb && ok) == 0) continue;
       T va[B], vb[B];
       int vdy[B];
 #pragma unroll
       for (int u = 0; u < B; ++u) {
         va[u] = vb[u] = T(0);
         vdy[u] = 0;
-        if (j0 + u < m) {
-          const int64_t o = (int64_t(y) * g.caps + j0 + u) * g.nx + c;
+        if (jb + u < m) {
+          const int64_t o = slot_index(g.ny, g.caps, c, y, jb + u);
           va[u] = g.m_lo[o];
           vb[u] = g.m_hi[o];
           vdy[u] = g.m_dy[o];
         }
       }
+      bool run = false;
+      double rs = 0, re = 0;
 #pragma unroll
       for (int u = 0; u < B; ++u) {
-        if (j0 + u >= m || !ok) continue;
+        if (jb + u >= m || !ok) continue;
         const int dy = vdy[u];
         double h;
         if (table) {
-          h = dy <= rint ? sm.h[dy * (rint + 1) + ad] : -1.0;
+          h = dy <= rint ? htab[dy * (rint + 1) + ad] : -1.0;
         } else {
           const double v = g.R2 - double(dy) * double(dy) - double(ad) * double(ad);
           h = v >= 0 ? sqrt(v) : -1.0;
@@ -955,40 +991,28 @@ __global__ __launch_bounds__(kWave) void pass2_gather_kernel(Pass2GArgs<T> g) {
           re = fmax(re, e);
           continue;
         }
-        if (run && !(rs >= clo && re <= chi)) {
-          int k = 0;
-          if (!lane_unite<OCAP>(sm.olo, sm.ohi, L, n, k, rs, re)) {
-            ok = false;
-            continue;
-          }
-          clo = sm.olo[k][L];
-          chi = sm.ohi[k][L];
-        }
+        if (run) unite(rs, re);
         run = true;
         rs = s;
         re = e;
       }
-    }
-    if (ok && run && !(rs >= clo && re <= chi)) {
-      int k = 0;
-      if (!lane_unite<OCAP>(sm.olo, sm.ohi, L, n, k, rs, re))
-        ok = false;
-      else {
-        clo = sm.olo[k][L];
-        chi = sm.ohi[k][L];
-      }
+      if (run && ok) unite(rs, re);
     }
   }
+  if (dirty && ok) {
+    olo[k * kWave] = clo;
+    ohi[k * kWave] = chi;
+  }
   if (ballot(active && !ok) != 0) {
-    if (L == 0) g.tile_list[1 + atomicAdd(g.tile_list, 1)] = blockIdx.x;
+    if (L == 0) g.tile_list[1 + atomicAdd(g.tile_list, 1)] = int(tile);
     return;
   }
   if (!active) return;
-  const int64_t col = rowbase + x;
-  const int64_t ob = int64_t(blockIdx.x) * g.arena + int64_t(L) * 2 * OCAP;
+  const int64_t col = int64_t(x) + int64_t(g.nx) * y;
+  const int64_t ob = tile * g.arena + int64_t(L) * 2 * ocap;
   for (int i = 0; i < n; ++i) {
-    g.o_z[ob + 2 * i] = T(sm.olo[i][L]);
-    g.o_z[ob + 2 * i + 1] = T(sm.ohi[i][L]);
+    g.o_z[ob + 2 * i] = T(olo[i * kWave]);
+    g.o_z[ob + 2 * i + 1] = T(ohi[i * kWave]);
   }
   g.o_start[col] = ob;
   g.o_cnt[col] = 2 * n;
@@ -1255,12 +1279,12 @@ __global__ __launch_bounds__(kWave) void pass1_paint_kernel(Pass1Args<T> g) {
       return;
     }
     for (int i = L; i < nu; i += kWave) {
-      const int64_t o = slot_index(g.nx, g.caps, x, y, i);
+      const int64_t o = slot_index(g.ny, g.caps, x, y, i);
       g.m_lo[o] = sm.ulo[i];
       g.m_hi[o] = sm.uhi[i];
       g.m_dy[o] = uint16_t(-sm.ukey[i]);
     }
-    if (L == 0) g.m_cnt[column(y)] = nu;
+    if (L == 0) g.m_cnt[count_index(g.ny, x, y)] = nu;
   }
 }
 
@@ -1326,69 +1350,77 @@ __device__ __forceinline__ void column_segments(const T* z, int64_t s, int64_t n
   f(ca, cb);
 }
 
-// Sweep state of one lane: coverage bits of the window columns (five 64-bit
-// words, held in registers), the distance to the nearest covered column and
+// Sweep state of one lane: coverage bits of the 2R + 1 window columns within
+// reach (bit k <-> window column L + k, so the lane's own column is bit R),
+// held in NW 64-bit registers, the distance to the nearest covered column and
 // the open piece.
+template <int NW>
 struct ZState {
-  uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0, m4 = 0;
-  int p = 0, R = 0;
+  uint64_t w[NW];
+  int R = 0;
   int dist = 1 << 20, dcur = 1 << 20, count = 0;
   double z0 = 0;
   bool point_open = false, ok = true;
 
-  __device__ __forceinline__ uint64_t word(int i) const {
-    return i == 0 ? m0 : i == 1 ? m1 : i == 2 ? m2 : i == 3 ? m3 : i == 4 ? m4 : 0;
+  __device__ __forceinline__ explicit ZState(int r) : R(r) {
+#pragma unroll
+    for (int t = 0; t < NW; ++t) w[t] = 0;
   }
-  __device__ __forceinline__ void flip(int w, bool on) {
-    const uint64_t b = uint64_t(1) << (w & 63);
-    const int i = w >> 6;
-    if (on) {
-      m0 |= i == 0 ? b : 0;
-      m1 |= i == 1 ? b : 0;
-      m2 |= i == 2 ? b : 0;
-      m3 |= i == 3 ? b : 0;
-      m4 |= i == 4 ? b : 0;
-    } else {
-      m0 &= i == 0 ? ~b : ~uint64_t(0);
-      m1 &= i == 1 ? ~b : ~uint64_t(0);
-      m2 &= i == 2 ? ~b : ~uint64_t(0);
-      m3 &= i == 3 ? ~b : ~uint64_t(0);
-      m4 &= i == 4 ? ~b : ~uint64_t(0);
+  __device__ __forceinline__ uint64_t word(int i) const {
+    uint64_t v = 0;
+#pragma unroll
+    for (int t = 0; t < NW; ++t) v = t == i ? w[t] : v;
+    return v;
+  }
+  __device__ __forceinline__ void flip(int k, bool on) {
+    const uint64_t b = uint64_t(1) << (k & 63);
+    const int i = k >> 6;
+#pragma unroll
+    for (int t = 0; t < NW; ++t) {
+      if (t == i) w[t] = on ? (w[t] | b) : (w[t] & ~b);
     }
   }
-  // distance from p to the nearest set bit within R, or R + 1
+  // distance from bit R to the nearest set bit, or R + 1 when none is set
   __device__ __forceinline__ int nearest() const {
     int best = R + 1;
-    for (int q = p, span = 0; span <= R && q < 5 * 64;) {
-      const int b = q & 63;
-      const uint64_t w = word(q >> 6) >> b;
-      if (w) {
-        best = min(best, span + __builtin_ctzll(w));
-        break;
+    if constexpr (NW == 1) {
+      const uint64_t up = w[0] >> R, dn = w[0] << (63 - R);
+      if (up) best = __builtin_ctzll(up);
+      if (dn) best = min(best, __builtin_clzll(dn));
+    } else {
+      for (int q = R, span = 0; span <= R && q < NW * 64;) {
+        const int b = q & 63;
+        const uint64_t v = word(q >> 6) >> b;
+        if (v) {
+          best = min(best, span + __builtin_ctzll(v));
+          break;
+        }
+        span += 64 - b;
+        q += 64 - b;
       }
-      span += 64 - b;
-      q += 64 - b;
-    }
-    for (int q = p, span = 0; span <= R && q >= 0;) {
-      const int b = q & 63;
-      const uint64_t w = word(q >> 6) << (63 - b);
-      if (w) {
-        best = min(best, span + __builtin_clzll(w));
-        break;
+      for (int q = R, span = 0; span <= R && q >= 0;) {
+        const int b = q & 63;
+        const uint64_t v = word(q >> 6) << (63 - b);
+        if (v) {
+          best = min(best, span + __builtin_clzll(v));
+          break;
+        }
+        span += b + 1;
+        q -= b + 1;
       }
-      span += b + 1;
-      q -= b + 1;
     }
     return best;
   }
-  __device__ __forceinline__ void set(int w) {
-    flip(w, true);
-    const int dd = w > p ? w - p : p - w;
+  __device__ __forceinline__ void set(int k) {
+    if (unsigned(k) > unsigned(2 * R)) return;
+    flip(k, true);
+    const int dd = k > R ? k - R : R - k;
     if (dd < dist) dist = dd;
   }
-  __device__ __forceinline__ void clear(int w) {
-    flip(w, false);
-    const int dd = w > p ? w - p : p - w;
+  __device__ __forceinline__ void clear(int k) {
+    if (unsigned(k) > unsigned(2 * R)) return;
+    flip(k, false);
+    const int dd = k > R ? k - R : R - k;
     if (dd == dist) dist = nearest();
   }
   // close the open piece and open the next one when the distance changed;
@@ -1398,7 +1430,7 @@ struct ZState {
     if (dist == dcur) return;
     if (dcur <= R && (z0 < z || point_open) && active) {
       if (count < g.caps) {
-        const int64_t o = slot_index(g.nx, g.caps, x, y, count);
+        const int64_t o = slot_index(g.ny, g.caps, x, y, count);
         g.m_lo[o] = T(z0);
         g.m_hi[o] = T(z);
         g.m_dy[o] = uint16_t(dcur);
@@ -1415,14 +1447,14 @@ struct ZState {
   }
 };
 
-template <class T, int ECAP, int NB, int WMAX>
+template <class T, int ECAP, int NB, int WMAX, int NW>
 __global__ __launch_bounds__(kWave) void pass1_zsweep_kernel(Pass1ZArgs<T> g) {
   using S = Pass1ZSmem<T, ECAP, NB, WMAX>;
-  static_assert(WMAX <= 5 * 64, "the sweep state holds five mask words");
   __shared__ S sm;
   const int L = lane_id();
-  const int x = blockIdx.x % g.nx;
-  const int y0 = (blockIdx.x / g.nx) * kZTile;
+  const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int x = int(tile % g.nx);
+  const int y0 = int(tile / g.nx) * kZTile;
   if (y0 >= g.ny) return;
   const int R = g.rint;
   const int wlo = y0 - R, wn = kZTile + 2 * R;
@@ -1449,10 +1481,9 @@ __global__ __launch_bounds__(kWave) void pass1_zsweep_kernel(Pass1ZArgs<T> g) {
     zlo = fmin(zlo, __shfl_xor(zlo, d));
     zhi = fmax(zhi, __shfl_xor(zhi, d));
   }
-  const int64_t outcol0 = int64_t(x) + int64_t(g.nx) * (y0 + L);
   const bool active = y0 + L < g.ny;
   if (!(zlo <= zhi)) {  // no input anywhere in the window
-    if (active) g.m_cnt[outcol0] = 0;
+    if (active) g.m_cnt[count_index(g.ny, x, y0 + L)] = 0;
     return;
   }
   const double scale = double(NB) / fmax(zhi - zlo, 1e-30) * (1.0 - 1e-12);
@@ -1508,9 +1539,11 @@ __global__ __launch_bounds__(kWave) void pass1_zsweep_kernel(Pass1ZArgs<T> g) {
     return;
   }
   // per-lane sweep state (lane = output column y0 + L, window position L + R)
-  ZState st;
-  st.p = L + R;
-  st.R = R;
+  if (2 * R + 1 > NW * 64) {  // the host picks NW from R; never taken
+    raise_status(g.status, kStatusP1List);
+    return;
+  }
+  ZState<NW> st(R);
   int b_lo = 0;
   for (int sl = 0; sl < nslab; ++sl) {
     const int b_hi = sm.slab_end[sl];
@@ -1597,9 +1630,9 @@ __global__ __launch_bounds__(kWave) void pass1_zsweep_kernel(Pass1ZArgs<T> g) {
         const int pk = __builtin_amdgcn_readlane(vp, i);
         const int kind = pk & 3;
         if (kind == 2)
-          st.clear(pk >> 2);
+          st.clear((pk >> 2) - L);
         else
-          st.set(pk >> 2);
+          st.set((pk >> 2) - L);
         double zn = 0;
         int kn = -1;
         if (i + 1 < cnt) {
@@ -1612,7 +1645,7 @@ __global__ __launch_bounds__(kWave) void pass1_zsweep_kernel(Pass1ZArgs<T> g) {
         if (kn < 0 || zn != z || kn != kind) {
           st.emit(z, kind == 1, active, g, x, y0 + L);
           if (kind == 1) {
-            for (int t = gstart; t <= base + i; ++t) st.clear(sm.ep[t] >> 2);
+            for (int t = gstart; t <= base + i; ++t) st.clear((sm.ep[t] >> 2) - L);
             st.emit(z, false, active, g, x, y0 + L);
           }
           gstart = base + i + 1;
@@ -1622,7 +1655,7 @@ __global__ __launch_bounds__(kWave) void pass1_zsweep_kernel(Pass1ZArgs<T> g) {
     b_lo = b_hi;
   }
   if (ballot(active && !st.ok) != 0) raise_status(g.status, kStatusArenaOverflow);
-  if (active) g.m_cnt[outcol0] = st.count;
+  if (active) g.m_cnt[count_index(g.ny, x, y0 + L)] = st.count;
 }
 
 }  // namespace vo

@@ -24,6 +24,18 @@ __device__ __forceinline__ int count_below(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
 }
 __device__ __forceinline__ int popc(uint64_t m) { return __popcll(m); }
+
+// Workgroup b of a launch runs on XCD b % 8 (the dispatcher deals workgroups
+// to the eight XCDs round-robin).  xcd_tile maps b to a logical tile so that
+// each XCD receives one contiguous range of tiles: neighbouring tiles, which
+// read and write the same cache lines, then meet in the same L2.  A
+// bijection on [0, n).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t n) {
+  const int64_t q = n / kXcds, r = n % kXcds;
+  const int64_t xcd = b % kXcds, k = b / kXcds;
+  return xcd * q + (xcd < r ? xcd : r) + k;
+}
 __device__ __forceinline__ void wave_sync() { __syncthreads(); }
 
 // Status bits written by the kernels (host turns them into error codes).

@@ -107,9 +107,9 @@ class CompressedVolume:
     # -- text format (CompressedVolume.cpp:116-152) -------------------------
     def save(self, path):
         with open(path, "w") as f:
-            f.write(f"{self.origin[0]!r} {self.origin[1]!r} {self.origin[2]!r}\n")
-            f.write(f"{self.extent[0]!r} {self.extent[1]!r} {self.extent[2]!r}\n")
-            f.write(f"{self.nx} {self.ny}\n{self.padding}\n{self.spacing!r}\n")
+            f.write("%r %r %r\n" % tuple(map(float, self.origin)))
+            f.write("%r %r %r\n" % tuple(map(float, self.extent)))
+            f.write(f"{self.nx} {self.ny}\n{self.padding}\n{float(self.spacing)!r}\n")
             for c in range(self.ncols):
                 col = self.column(c)
                 f.write(str(len(col)) + "".join(" " + repr(float(v)) for v in col) + "\n")
