@@ -75,6 +75,29 @@ void vo_result_free(vo_result* r);
 int vo_morph_device(vo_context* ctx, int op, int precision, double radius, const vo_volume* in,
                     vo_volume* out, int64_t* out_count, double* time_1_ms, double* time_2_ms);
 
+/* Batched form (no single reference counterpart: it replaces a loop of
+ * VoronoiMorpho::dilation / erosion calls and the CLI's closing / opening,
+ * Voronoi.h:18-29, offset3d.cpp:116-136, over many volumes).  Each job names
+ * an op, a radius and a device volume whose z has the batch's precision; all
+ * jobs run together, one launch per pass and stage for the whole batch.
+ * out[i] receives device pointers into the context's workspace (offsets
+ * starting at 0), valid until the next call on `ctx`; out_count[i] is its
+ * number of endpoints.  time_1 / time_2 sum the batched pass-1 / pass-2
+ * kernels.  Volumes whose working lists outgrow the batched kernels are redone
+ * one by one with the single-volume fallbacks; results do not depend on it. */
+typedef struct {
+  int op;         /* vo_op */
+  double radius;  /* dexels */
+  vo_volume in;   /* device pointers */
+} vo_job;
+
+int vo_morph_batch_device(vo_context* ctx, int precision, int n, const vo_job* jobs, vo_volume* out,
+                          int64_t* out_count, double* time_1_ms, double* time_2_ms);
+
+/* Number of jobs of the last batch on `ctx` that were redone one by one
+ * (a tuning signal).  No reference counterpart. */
+int vo_last_batch_alone(const vo_context* ctx, int* jobs);
+
 /* Mesh -> dexels: replaces the ray casting of vor3d::create_dexels
  * (src/vor3d/Dexelize.cpp:166-225; predicates :56-162).  `vertices` is
  * num_vertices x 3 doubles (x, y, z), `triangles` num_triangles x 3 vertex

@@ -1,0 +1,87 @@
+"""Batched form (vo_morph_batch_device): many volumes of uneven size, radius
+and operation in one batch, checked against the single-volume path (the same
+kernels, so bit-identical) and against the oracle."""
+
+import numpy as np
+import pytest
+
+import oracle
+from voroffset_amd import synth
+from voroffset_amd.volume import CompressedVolume
+
+from .helpers import compare_csr, narrow_gap_mismatches, quantize_fp32
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_items(seed, n=10, side=(12, 60), r=(1.0, 9.0)):
+    rng = np.random.default_rng(seed)
+    items = []
+    for k in range(n):
+        nx, ny = int(rng.integers(*side)), int(rng.integers(*side))
+        vol = synth.poisson_columns(nx, ny, lam=float(rng.uniform(2, 8)), zmax=300, seed=seed * 100 + k, z_margin=14)
+        items.append((synth.OPS[k % 4], vol, float(rng.uniform(*r))))
+    return items
+
+
+def test_batch_equals_single_path_fp64(hip_fp64):
+    items = mixed_items(1)
+    res, t1, t2 = hip_fp64.apply_batch(items)
+    assert t1 > 0 and t2 > 0
+    for (op, vol, r), got in zip(items, res):
+        one, _, _ = hip_fp64.apply(op, vol, r)
+        assert (got.nx, got.ny) == (vol.nx, vol.ny)
+        assert np.array_equal(got.off, one.off), op
+        assert np.array_equal(got.z, one.z), op
+        bad, err = compare_csr(got.off, got.z, *oracle.morph(vol, op, r))
+        assert len(bad) == 0 and err <= 1e-9, (op, r, bad[:5], err)
+
+
+def test_batch_fp32_against_oracle(hip_fp32):
+    items = mixed_items(2, n=12)
+    res, _, _ = hip_fp32.apply_batch(items)
+    for (op, vol, r), got in zip(items, res):
+        ref = oracle.morph(quantize_fp32(vol), op, r)
+        bad, err = compare_csr(got.off, got.z, *ref)
+        assert narrow_gap_mismatches(got.off, got.z, ref[0], ref[1], bad) and err <= 1e-4, (op, r, bad[:5], err)
+
+
+def _vol(nx, ny, cols, zext):
+    return CompressedVolume.from_columns(nx, ny, cols, origin=(0.0, 0.0, 0.0),
+                                         extent=(float(nx), float(ny), zext), spacing=1.0, padding=0)
+
+
+def test_batch_jobs_redone_alone(hip_fp64):
+    # 200 segments in one column at r = 0.5 overflow the batched gather lists;
+    # r = 130 is beyond the batched pass-1 form: both go through the
+    # single-volume fallbacks inside the same call
+    dense = _vol(5, 5, [[float(k) for k in range(0, 400, 2)] if i == 12 else [] for i in range(25)], 400.0)
+    wide = synth.poisson_columns(9, 7, lam=3, zmax=200, seed=5, z_margin=140)
+    plain = synth.poisson_columns(30, 20, lam=5, zmax=200, seed=6)
+    items = [("dilation", dense, 0.5), ("dilation", wide, 130.0), ("dilation", plain, 4.0), ("closing", plain, 2.0)]
+    res, _, _ = hip_fp64.apply_batch(items)
+    for (op, vol, r), got in zip(items, res):
+        bad, err = compare_csr(got.off, got.z, *oracle.morph(vol, op, r))
+        assert len(bad) == 0 and err <= 1e-9, (op, r, bad[:5], err)
+    assert hip_fp64.ctx.last_batch_alone() >= 2
+
+
+def test_empty_batch_and_empty_grid(hip_fp64):
+    res, t1, t2 = hip_fp64.apply_batch([])
+    assert res == []
+    empty = _vol(0, 0, [], 10.0)
+    blank = _vol(6, 4, [[] for _ in range(24)], 10.0)
+    res, _, _ = hip_fp64.apply_batch([("dilation", empty, 2.0), ("erosion", blank, 1.0), ("noop", blank, 1.0)])
+    assert res[0].z.size == 0 and res[1].z.size == 0 and res[2].z.size == 0
+
+
+def test_batch_uneven_specs_fp64(hip_fp64):
+    """The uneven-batch generator (sides log-uniform, lambda U[2,10], r U[2,32],
+    ops uniform), at reduced sides so the oracle finishes quickly."""
+    specs = synth.batch_specs(16, seed=77, side_range=(24, 72), r_range=(2.0, 12.0))
+    items = [(s["op"], synth.poisson_columns(s["side"], s["side"], lam=s["lam"], zmax=256, seed=s["seed"],
+                                             z_margin=int(s["radius"]) + 4), s["radius"]) for s in specs]
+    res, _, _ = hip_fp64.apply_batch(items)
+    for (op, vol, r), got in zip(items, res):
+        bad, err = compare_csr(got.off, got.z, *oracle.morph(vol, op, r, threads=4))
+        assert len(bad) == 0 and err <= 1e-9, (op, r, vol.nx, bad[:5], err)

@@ -20,7 +20,8 @@ PRECISIONS = {"fp32": 0, "fp64": 1}
 
 # every symbol the header declares
 EXPORTS = ("vo_create", "vo_destroy", "vo_last_error", "vo_morph", "vo_result_shape", "vo_result_copy",
-           "vo_result_free", "vo_morph_device", "vo_last_stats", "vo_rasterize")
+           "vo_result_free", "vo_morph_device", "vo_last_stats", "vo_rasterize", "vo_morph_batch_device",
+           "vo_last_batch_alone")
 
 
 class VoError(RuntimeError):
@@ -32,6 +33,10 @@ class VoError(RuntimeError):
 class VoVolume(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("offsets", ctypes.c_void_p),
                 ("z", ctypes.c_void_p), ("z_lo", ctypes.c_double), ("z_hi", ctypes.c_double)]
+
+
+class VoJob(ctypes.Structure):
+    _fields_ = [("op", ctypes.c_int), ("radius", ctypes.c_double), ("vol", VoVolume)]
 
 
 _lib = None
@@ -67,6 +72,11 @@ def lib():
         L.vo_last_stats.restype = i32
         L.vo_rasterize.argtypes = [vp, vp, i64, vp, i64, i32, i32, dbl, dbl, dbl, ctypes.POINTER(vp)]
         L.vo_rasterize.restype = i32
+        L.vo_morph_batch_device.argtypes = [vp, i32, i32, ctypes.POINTER(VoJob), ctypes.POINTER(VoVolume),
+                                            ctypes.POINTER(i64), ctypes.POINTER(dbl), ctypes.POINTER(dbl)]
+        L.vo_morph_batch_device.restype = i32
+        L.vo_last_batch_alone.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+        L.vo_last_batch_alone.restype = i32
         _lib = L
     return _lib
 
@@ -128,6 +138,23 @@ class Context:
             self._raise(rc)
         return vout, cnt.value, (t1.value, t2.value)
 
+    def morph_batch_device(self, precision, jobs):
+        """Batched form.  jobs: sequence of (op, radius, nx, ny, off_ptr, z_ptr,
+        z_lo, z_hi) with device pointers.  Returns ([(VoVolume, count)],
+        (time_1, time_2)); outputs live in the workspace until the next call."""
+        n = len(jobs)
+        arr = (VoJob * max(n, 1))()
+        for i, (op, r, nx, ny, off, z, zlo, zhi) in enumerate(jobs):
+            arr[i] = VoJob(OP_CODES[op], float(r), VoVolume(nx, ny, off, z, zlo, zhi))
+        outs = (VoVolume * max(n, 1))()
+        cnt = (ctypes.c_int64 * max(n, 1))()
+        t1, t2 = ctypes.c_double(0), ctypes.c_double(0)
+        rc = lib().vo_morph_batch_device(self._h, PRECISIONS[precision], n, arr, outs, cnt, ctypes.byref(t1),
+                                         ctypes.byref(t2))
+        if rc != OK:
+            self._raise(rc)
+        return [(outs[i], cnt[i]) for i in range(n)], (t1.value, t2.value)
+
     def rasterize(self, vertices, triangles, nx, ny, origin_x, origin_y, spacing):
         """Mesh -> dexel columns (CSR offsets int64, z float64 in units of spacing)."""
         V = np.ascontiguousarray(vertices, np.float64).reshape(-1, 3)
@@ -150,6 +177,12 @@ class Context:
         finally:
             lib().vo_result_free(res)
         return o, zz
+
+    def last_batch_alone(self):
+        """Jobs of the last batch that were redone by the single-volume path."""
+        n = ctypes.c_int(0)
+        lib().vo_last_batch_alone(self._h, ctypes.byref(n))
+        return n.value
 
     def last_stats(self):
         """(output endpoints, pass-2 tiles recomputed by the sweep form) of the last dilation."""

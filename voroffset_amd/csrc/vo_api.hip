@@ -80,7 +80,9 @@ enum Slot {
   kV0_OFF, kV0_Z, kV1_OFF, kV1_Z, kV2_OFF, kV2_Z, kV3_OFF, kV3_Z,  // device volumes
   kIN_OFF, kIN_Z,                                                   // staged host input
   kR_V, kR_F, kR_TCNT, kR_TOFF, kR_TCUR, kR_TTRIS, kR_LEN, kR_OFF, kR_Z,  // rasterizer
-  kSLOTS
+  kB_TAB0, kB_TAB1, kB_TAB2, kB_STATUS, kB_FIN_OFF, kB_FIN_AUX,         // batched form
+  kB_OUT,                                  // + (stage * 3 + kind) * 2 (+1 for z): stage outputs
+  kSLOTS = kB_OUT + 4 * 3 * 2
 };
 
 struct DevVol {
@@ -105,6 +107,8 @@ struct vo_context {
   bool force_sweep = false;  // VO_PASS2=sweep: always use the sweep form of pass 2
   int pass1_form = 0;        // VO_PASS1: 0 = z-sweep (default), 1 = "paint", 2 = "cull"
   int last_fallback_tiles = 0;
+  std::vector<void*> batch_extra;  // outputs of batched jobs redone alone (freed by the next batch)
+  int last_batch_alone = 0;        // jobs of the last batch redone by the single-volume path
 };
 
 struct vo_result {
@@ -264,7 +268,7 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
     int nfall = 0;
     if (gather) {
       vo::Pass2GArgs<T> ag{nx, ny, rint, R2, m_lo, m_hi, m_dy, caps, m_cnt, o_z, garena, o_start, o_len,
-                           tiles, status, gocap, gtable ? 1 : 0};
+                           tiles, status, gocap, gtable ? 1 : 0, 0};
       const size_t lds = vo::gather_lds_bytes(rint, gocap, gtable);
       hipLaunchKernelGGL((vo::pass2_gather_kernel<T>), dim3(unsigned(ntiles)), dim3(vo::kWave), lds,
                          ctx->stream, ag);
@@ -335,12 +339,12 @@ DevVol column_transform(vo_context* ctx, bool ring, const DevVol& in, T lo, T hi
   int64_t* off = static_cast<int64_t*>(ws(ctx, off_slot, (n + 1) * sizeof(int64_t)));
   auto launch = [&](int phase, const int64_t* o, T* z) {
     const dim3 grid(blocks_for(std::max<int64_t>(n, 1), 256));
+    vo::ColumnJob<T> job{ring ? in.nx : onx, ring ? in.ny : ony, in.off, static_cast<const T*>(in.z), lo, hi, len,
+                         o, z};
     if (ring)
-      hipLaunchKernelGGL(vo::ring_complement_kernel<T>, grid, dim3(256), 0, ctx->stream, in.nx, in.ny, in.off,
-                         static_cast<const T*>(in.z), lo, hi, phase, len, o, z);
+      hipLaunchKernelGGL(vo::ring_complement_kernel<T>, grid, dim3(256), 0, ctx->stream, job, phase);
     else
-      hipLaunchKernelGGL(vo::crop_complement_kernel<T>, grid, dim3(256), 0, ctx->stream, onx, ony, in.off,
-                         static_cast<const T*>(in.z), lo, hi, phase, len, o, z);
+      hipLaunchKernelGGL(vo::crop_complement_kernel<T>, grid, dim3(256), 0, ctx->stream, job, phase);
     check(ctx, hipGetLastError(), "complement launch");
   };
   launch(0, nullptr, nullptr);
@@ -388,6 +392,311 @@ DevVol run_op(vo_context* ctx, int op, const DevVol& in, double R, double zlo, d
       fail(ctx, VO_ERR_ARGUMENT, "unknown operation");
   }
   return in;
+}
+
+void validate_shape(vo_context* ctx, const vo_volume* v);
+
+// ---------------------------------------------------------------------------
+// Batched form: many volumes per launch.  Every primitive (ring complement,
+// dilation, crop complement) runs once per stage for all the volumes that
+// need it; a volume whose lists overflow in the batched kernels is redone
+// alone by the single-volume path, which has the sweep fallbacks.
+
+template <class V>
+V* upload_table(vo_context* ctx, int slot, const std::vector<V>& host) {
+  V* d = static_cast<V*>(ws(ctx, slot, host.size() * sizeof(V)));
+  check(ctx, hipMemcpyAsync(d, host.data(), host.size() * sizeof(V), hipMemcpyHostToDevice, ctx->stream),
+        "upload table");
+  return d;
+}
+
+// Keeps a copy of a single-volume result that must outlive later calls.
+template <class T>
+DevVol keep_alone(vo_context* ctx, const DevVol& r) {
+  const int64_t ncols = int64_t(r.nx) * r.ny;
+  const int64_t count = read_word(ctx, r.off + ncols);  // 0-based offsets
+  void *off = nullptr, *z = nullptr;
+  check(ctx, hipMalloc(&off, size_t(ncols + 1) * sizeof(int64_t)), "hipMalloc");
+  ctx->batch_extra.push_back(off);
+  check(ctx, hipMalloc(&z, size_t(std::max<int64_t>(count, 1)) * sizeof(T)), "hipMalloc");
+  ctx->batch_extra.push_back(z);
+  check(ctx, hipMemcpyAsync(off, r.off, size_t(ncols + 1) * sizeof(int64_t), hipMemcpyDeviceToDevice, ctx->stream),
+        "copy");
+  if (count)
+    check(ctx, hipMemcpyAsync(z, r.z, size_t(count) * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream), "copy");
+  DevVol k = r;
+  k.off = static_cast<const int64_t*>(off);
+  k.z = z;
+  k.count = count;
+  return k;
+}
+
+// Dilation of the volumes vol[idx[...]] by R[...]: results replace vol[...]
+// (offsets absolute into one shared endpoint buffer).
+template <class T>
+void dilate_batch(vo_context* ctx, std::vector<DevVol>& vol, const std::vector<double>& R,
+                  const std::vector<int>& idx, int off_slot, int z_slot, double* t1, double* t2) {
+  std::vector<int> jobs, alone;
+  for (int j : idx) {
+    if (!(R[j] >= 0) || !std::isfinite(R[j]) || R[j] > 60000) fail(ctx, VO_ERR_ARGUMENT, "radius must be finite and >= 0");
+    const int rint = int(std::floor(R[j]));
+    if (rint > 127 || int64_t(vol[j].nx) * vol[j].ny == 0 || ctx->force_sweep || ctx->pass1_form != 0)
+      alone.push_back(j);
+    else
+      jobs.push_back(j);
+  }
+  const int K = int(jobs.size());
+  if (K > 0) {
+    const int caps = ctx->p1_caps;
+    std::vector<int64_t> col_base(K + 1, 0), tile_base(K + 1, 0);
+    int max_rint = 0, ocap = 0;
+    for (int k = 0; k < K; ++k) {
+      const DevVol& v = vol[jobs[k]];
+      const int rint = int(std::floor(R[jobs[k]]));
+      col_base[k + 1] = col_base[k] + int64_t(v.nx) * v.ny;
+      tile_base[k + 1] = tile_base[k] + int64_t(v.nx) * ((v.ny + vo::kGatherTile - 1) / vo::kGatherTile);
+      max_rint = std::max(max_rint, rint);
+      ocap = std::max(ocap, gather_ocap(rint));
+    }
+    size_t lds = 0;
+    for (int k = 0; k < K; ++k) {
+      const int rint = int(std::floor(R[jobs[k]]));
+      lds = std::max(lds, vo::gather_lds_bytes(rint, ocap, rint <= kGHT));
+    }
+    const int64_t C = col_base[K], NT = tile_base[K];
+    if (NT > INT32_MAX || C + 1 > INT32_MAX) fail(ctx, VO_ERR_ARGUMENT, "batch too large");
+    const int64_t garena = int64_t(vo::kGatherTile) * 2 * ocap;
+    T* m_lo = static_cast<T*>(ws(ctx, kM_LO, size_t(C) * caps * sizeof(T)));
+    T* m_hi = static_cast<T*>(ws(ctx, kM_HI, size_t(C) * caps * sizeof(T)));
+    uint16_t* m_dy = static_cast<uint16_t*>(ws(ctx, kM_DY, size_t(C) * caps * sizeof(uint16_t)));
+    int32_t* m_cnt = static_cast<int32_t*>(ws(ctx, kM_CNT, size_t(C) * sizeof(int32_t)));
+    T* o_z = static_cast<T*>(ws(ctx, kO_Z, size_t(NT) * garena * sizeof(T)));
+    int64_t* o_start = static_cast<int64_t*>(ws(ctx, kO_START, size_t(C) * sizeof(int64_t)));
+    int64_t* o_len = static_cast<int64_t*>(ws(ctx, kO_LEN, size_t(C + 1) * sizeof(int64_t)));
+    int* status = static_cast<int*>(ws(ctx, kB_STATUS, size_t(K) * sizeof(int)));
+    std::vector<vo::Pass1ZArgs<T>> a1(K);
+    std::vector<vo::Pass2GArgs<T>> a2(K);
+    for (int k = 0; k < K; ++k) {
+      const DevVol& v = vol[jobs[k]];
+      const int rint = int(std::floor(R[jobs[k]]));
+      const int64_t cb = col_base[k], sb = cb * caps;
+      a1[k] = vo::Pass1ZArgs<T>{v.nx, v.ny, rint, v.off, static_cast<const T*>(v.z), m_lo + sb, m_hi + sb, m_dy + sb,
+                                caps, m_cnt + cb, status + k};
+      a2[k] = vo::Pass2GArgs<T>{v.nx, v.ny, rint, R[jobs[k]] * R[jobs[k]], m_lo + sb, m_hi + sb, m_dy + sb, caps,
+                                m_cnt + cb, o_z, garena, o_start + cb, o_len + cb, nullptr, status + k, ocap,
+                                rint <= kGHT ? 1 : 0, tile_base[k] * garena};
+    }
+    const auto* d1 = upload_table(ctx, kB_TAB0, a1);
+    const auto* d2 = upload_table(ctx, kB_TAB1, a2);
+    const int64_t* dt = upload_table(ctx, kB_TAB2, tile_base);
+    check(ctx, hipMemsetAsync(status, 0, size_t(K) * sizeof(int), ctx->stream), "memset status");
+    // columns of a volume whose pass 1 failed, or of a gather tile that
+    // overflowed, are never written: zero lengths keep the batch-wide CSR
+    // assembly from reading them (those volumes are redone alone)
+    check(ctx, hipMemsetAsync(o_len, 0, size_t(C + 1) * sizeof(int64_t), ctx->stream), "memset lengths");
+    check(ctx, hipEventRecord(ctx->ev[0], ctx->stream), "event");
+    const dim3 grid(static_cast<unsigned>(NT)), block(static_cast<unsigned>(vo::kWave));
+    if (max_rint <= 31)
+      hipLaunchKernelGGL((vo::pass1_zsweep_batch_kernel<T, kZECap, kZNB, kZWMax, 1>), grid, block, 0, ctx->stream, d1,
+                         dt, K);
+    else if (max_rint <= 63)
+      hipLaunchKernelGGL((vo::pass1_zsweep_batch_kernel<T, kZECap, kZNB, kZWMax, 2>), grid, block, 0, ctx->stream, d1,
+                         dt, K);
+    else
+      hipLaunchKernelGGL((vo::pass1_zsweep_batch_kernel<T, kZECap, kZNB, kZWMax, 4>), grid, block, 0, ctx->stream, d1,
+                         dt, K);
+    check(ctx, hipGetLastError(), "batched pass1 launch");
+    check(ctx, hipEventRecord(ctx->ev[1], ctx->stream), "event");
+    hipLaunchKernelGGL((vo::pass2_gather_batch_kernel<T>), grid, block, lds, ctx->stream, d2, dt, K);
+    check(ctx, hipGetLastError(), "batched pass2 launch");
+    check(ctx, hipEventRecord(ctx->ev[2], ctx->stream), "event");
+    // output CSR over all columns of the batch
+    int64_t* glob = static_cast<int64_t*>(ws(ctx, off_slot, size_t(C + 1) * sizeof(int64_t)));
+    exclusive_scan(ctx, o_len, glob, C + 1);
+    std::vector<int> st(K);
+    check(ctx, hipMemcpyAsync(st.data(), status, size_t(K) * sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
+          "copy status");
+    const int64_t total = read_word(ctx, glob + C);  // synchronises
+    float ms1 = 0, ms2 = 0;
+    check(ctx, hipEventElapsedTime(&ms1, ctx->ev[0], ctx->ev[1]), "elapsed");
+    check(ctx, hipEventElapsedTime(&ms2, ctx->ev[1], ctx->ev[2]), "elapsed");
+    if (t1) *t1 += ms1;
+    if (t2) *t2 += ms2;
+    T* z = static_cast<T*>(ws(ctx, z_slot, size_t(std::max<int64_t>(total, 1)) * sizeof(T)));
+    hipLaunchKernelGGL(vo::gather_kernel<T>, dim3(blocks_for(C, 256)), dim3(256), 0, ctx->stream, C,
+                       static_cast<const T*>(o_z), o_start, o_len, glob, z);
+    check(ctx, hipGetLastError(), "gather launch");
+    for (int k = 0; k < K; ++k) {
+      DevVol& v = vol[jobs[k]];
+      if (st[k] != 0) {
+        alone.push_back(jobs[k]);  // lists overflowed: the single-volume path has the fallbacks
+        continue;
+      }
+      v.off = glob + col_base[k];
+      v.z = z;
+      v.count = -1;
+    }
+  }
+  for (int j : alone) {
+    double a = 0, b = 0;
+    DevVol r = dilate<T>(ctx, vol[j], R[j], kV3_OFF, kV3_Z, &a, &b);
+    if (t1) *t1 += a;
+    if (t2) *t2 += b;
+    vol[j] = keep_alone<T>(ctx, r);
+    ++ctx->last_batch_alone;
+  }
+}
+
+// Ring (grid + 2, complement within [lo, hi]) or crop (grid - 2, complement
+// within [lo, hi] with parity) of the volumes vol[idx[...]], in place.
+template <class T>
+void complement_batch(vo_context* ctx, bool ring, std::vector<DevVol>& vol, const std::vector<int>& idx,
+                      const std::vector<double>& lo, const std::vector<double>& hi, int off_slot, int z_slot) {
+  const int K = int(idx.size());
+  if (K == 0) return;
+  std::vector<int64_t> col_base(K + 1, 0);
+  for (int k = 0; k < K; ++k) {
+    const DevVol& v = vol[idx[k]];
+    if (!ring && (v.nx < 2 || v.ny < 2)) fail(ctx, VO_ERR_ARGUMENT, "crop of a grid smaller than 2x2");
+    col_base[k + 1] = col_base[k] + (ring ? int64_t(v.nx + 2) * (v.ny + 2) : int64_t(v.nx - 2) * (v.ny - 2));
+  }
+  const int64_t C = col_base[K];
+  if (C + 1 > INT32_MAX) fail(ctx, VO_ERR_ARGUMENT, "batch too large");
+  int64_t* len = static_cast<int64_t*>(ws(ctx, kO_LEN, size_t(C + 1) * sizeof(int64_t)));
+  int64_t* glob = static_cast<int64_t*>(ws(ctx, off_slot, size_t(C + 1) * sizeof(int64_t)));
+  std::vector<vo::ColumnJob<T>> t(K);
+  for (int k = 0; k < K; ++k) {
+    const DevVol& v = vol[idx[k]];
+    t[k] = vo::ColumnJob<T>{ring ? v.nx : v.nx - 2, ring ? v.ny : v.ny - 2, v.off, static_cast<const T*>(v.z),
+                            T(lo[idx[k]]), T(hi[idx[k]]), len + col_base[k], glob + col_base[k], nullptr};
+  }
+  const int64_t* dcb = upload_table(ctx, kB_TAB2, col_base);
+  const vo::ColumnJob<T>* dt = upload_table(ctx, kB_TAB0, t);
+  const dim3 grid(blocks_for(std::max<int64_t>(C, 1), 256));
+  auto launch = [&](int phase) {
+    if (ring)
+      hipLaunchKernelGGL(vo::ring_complement_batch_kernel<T>, grid, dim3(256), 0, ctx->stream, dt, dcb, K, phase);
+    else
+      hipLaunchKernelGGL(vo::crop_complement_batch_kernel<T>, grid, dim3(256), 0, ctx->stream, dt, dcb, K, phase);
+    check(ctx, hipGetLastError(), "batched complement launch");
+  };
+  launch(0);
+  check(ctx, hipMemsetAsync(len + C, 0, sizeof(int64_t), ctx->stream), "memset");
+  exclusive_scan(ctx, len, glob, C + 1);
+  const int64_t total = read_word(ctx, glob + C);  // synchronises: the first table upload is done
+  T* z = static_cast<T*>(ws(ctx, z_slot, size_t(std::max<int64_t>(total, 1)) * sizeof(T)));
+  for (auto& e : t) e.out = z;
+  dt = upload_table(ctx, kB_TAB0, t);
+  launch(1);
+  check(ctx, hipStreamSynchronize(ctx->stream), "sync");  // the host table must outlive its upload
+  for (int k = 0; k < K; ++k) {
+    DevVol& v = vol[idx[k]];
+    v.nx = ring ? v.nx + 2 : v.nx - 2;
+    v.ny = ring ? v.ny + 2 : v.ny - 2;
+    v.off = glob + col_base[k];
+    v.z = z;
+    v.count = -1;
+  }
+}
+
+// The programs of the four operations (Voronoi.cpp:8-17, offset3d.cpp:124-133).
+enum BatchStep { kStepNone = -1, kStepRing = 0, kStepDilate = 1, kStepCrop = 2 };
+
+int batch_step(int op, int s) {
+  static const int prog[5][4] = {
+      {kStepNone, kStepNone, kStepNone, kStepNone},         // noop
+      {kStepDilate, kStepNone, kStepNone, kStepNone},       // dilation
+      {kStepRing, kStepDilate, kStepCrop, kStepNone},       // erosion
+      {kStepDilate, kStepRing, kStepDilate, kStepCrop},     // closing
+      {kStepRing, kStepDilate, kStepCrop, kStepDilate}};    // opening
+  return prog[op][s];
+}
+
+template <class T>
+void run_batch(vo_context* ctx, const std::vector<int>& op, const std::vector<double>& R, std::vector<DevVol>& vol,
+               const std::vector<double>& zlo, const std::vector<double>& zhi, double* t1, double* t2) {
+  const int n = int(op.size());
+  std::vector<double> rlo(n), rhi(n);
+  for (int j = 0; j < n; ++j) {
+    rlo[j] = zlo[j] - 1;
+    rhi[j] = zhi[j] + 1;
+  }
+  for (int s = 0; s < 4; ++s) {
+    std::vector<int> by[3];
+    for (int j = 0; j < n; ++j) {
+      const int k = batch_step(op[j], s);
+      if (k != kStepNone) by[k].push_back(j);
+    }
+    const int base = kB_OUT + s * 3 * 2;
+    complement_batch<T>(ctx, true, vol, by[kStepRing], rlo, rhi, base, base + 1);
+    dilate_batch<T>(ctx, vol, R, by[kStepDilate], base + 2, base + 3, t1, t2);
+    complement_batch<T>(ctx, false, vol, by[kStepCrop], zlo, zhi, base + 4, base + 5);
+  }
+}
+
+void free_batch_extra(vo_context* ctx) {
+  if (ctx->batch_extra.empty()) return;
+  check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+  for (void* p : ctx->batch_extra) (void)hipFree(p);
+  ctx->batch_extra.clear();
+}
+
+template <class T>
+void morph_batch_device(vo_context* ctx, int n, const vo_job* jobs, vo_volume* out, int64_t* out_count, double* t1,
+                        double* t2) {
+  if (n < 0 || (n > 0 && (!jobs || !out))) fail(ctx, VO_ERR_ARGUMENT, "invalid batch");
+  free_batch_extra(ctx);
+  ctx->last_batch_alone = 0;
+  std::vector<int> op(n);
+  std::vector<double> R(n), zlo(n), zhi(n);
+  std::vector<DevVol> vol(n);
+  for (int j = 0; j < n; ++j) {
+    const vo_job& jb = jobs[j];
+    validate_shape(ctx, &jb.in);
+    if (jb.op < VO_OP_NOOP || jb.op > VO_OP_OPENING) fail(ctx, VO_ERR_ARGUMENT, "unknown operation");
+    if (!(jb.radius >= 0) || !std::isfinite(jb.radius)) fail(ctx, VO_ERR_ARGUMENT, "radius must be finite and >= 0");
+    if ((jb.op == VO_OP_EROSION || jb.op == VO_OP_OPENING || jb.op == VO_OP_CLOSING) && !(jb.in.z_lo <= jb.in.z_hi))
+      fail(ctx, VO_ERR_ARGUMENT, "z_lo must not exceed z_hi");
+    op[j] = jb.op;
+    R[j] = jb.radius;
+    zlo[j] = jb.in.z_lo;
+    zhi[j] = jb.in.z_hi;
+    vol[j].nx = jb.in.nx;
+    vol[j].ny = jb.in.ny;
+    vol[j].off = jb.in.offsets;
+    vol[j].z = jb.in.z;
+  }
+  if (t1) *t1 = 0;
+  if (t2) *t2 = 0;
+  run_batch<T>(ctx, op, R, vol, zlo, zhi, t1, t2);
+  if (n == 0) return;
+  // rebase every result's offsets to 0 in one buffer
+  std::vector<vo::FinalJob> fj(n);
+  std::vector<int64_t> entry_base(n + 1, 0);
+  for (int j = 0; j < n; ++j) {
+    const int64_t nc = int64_t(vol[j].nx) * vol[j].ny;
+    fj[j] = vo::FinalJob{vol[j].off, nc, entry_base[j]};
+    entry_base[j + 1] = entry_base[j] + nc + 1;
+  }
+  int64_t* rel = static_cast<int64_t*>(ws(ctx, kB_FIN_OFF, size_t(entry_base[n]) * sizeof(int64_t)));
+  int64_t* aux = static_cast<int64_t*>(ws(ctx, kB_FIN_AUX, size_t(2 * n) * sizeof(int64_t)));
+  const vo::FinalJob* dfj = upload_table(ctx, kB_TAB0, fj);
+  const int64_t* deb = upload_table(ctx, kB_TAB2, entry_base);
+  hipLaunchKernelGGL(vo::finalize_offsets_kernel, dim3(blocks_for(entry_base[n], 256)), dim3(256), 0, ctx->stream,
+                     dfj, deb, n, rel, aux, aux + n);
+  check(ctx, hipGetLastError(), "finalize launch");
+  std::vector<int64_t> h(size_t(2 * n));
+  check(ctx, hipMemcpyAsync(h.data(), aux, h.size() * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream), "copy");
+  check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+  for (int j = 0; j < n; ++j) {
+    out[j].nx = vol[j].nx;
+    out[j].ny = vol[j].ny;
+    out[j].offsets = rel + entry_base[j];
+    out[j].z = static_cast<const T*>(vol[j].z) + h[j];
+    out[j].z_lo = zlo[j];
+    out[j].z_hi = zhi[j];
+    if (out_count) out_count[j] = h[n + j];
+  }
 }
 
 int guarded(vo_context* ctx, const std::function<void()>& body) {
@@ -573,6 +882,7 @@ void vo_destroy(vo_context* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (void* p : ctx->batch_extra) (void)hipFree(p);
   for (auto& b : ctx->buf)
     if (b.ptr) (void)hipFree(b.ptr);
   for (auto& e : ctx->ev)
@@ -660,6 +970,24 @@ int vo_morph_device(vo_context* ctx, int op, int precision, double radius, const
     out->z_hi = in->z_hi;
     *out_count = r.count;
   });
+}
+
+int vo_morph_batch_device(vo_context* ctx, int precision, int n, const vo_job* jobs, vo_volume* out,
+                          int64_t* out_count, double* time_1_ms, double* time_2_ms) {
+  return guarded(ctx, [&] {
+    if (precision == VO_FP32)
+      morph_batch_device<float>(ctx, n, jobs, out, out_count, time_1_ms, time_2_ms);
+    else if (precision == VO_FP64)
+      morph_batch_device<double>(ctx, n, jobs, out, out_count, time_1_ms, time_2_ms);
+    else
+      fail(ctx, VO_ERR_ARGUMENT, "unknown precision");
+  });
+}
+
+int vo_last_batch_alone(const vo_context* ctx, int* jobs) {
+  if (!ctx || !jobs) return VO_ERR_ARGUMENT;
+  *jobs = ctx->last_batch_alone;
+  return VO_OK;
 }
 
 int vo_last_stats(const vo_context* ctx, int64_t* output_endpoints, int64_t* fallback_tiles) {

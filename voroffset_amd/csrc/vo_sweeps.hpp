@@ -65,6 +65,20 @@ __device__ __forceinline__ int64_t slot_index(int ny, int caps, int x, int y, in
 }
 __device__ __forceinline__ int64_t count_index(int ny, int x, int y) { return int64_t(x) * ny + y; }
 
+// Batched launches: job j owns the global tiles (or columns)
+// [base[j], base[j + 1]); returns the j holding t (base[njobs] = total).
+__device__ __forceinline__ int find_job(const int64_t* base, int njobs, int64_t t) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (base[mid] <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
 template <class T, int CAP, int NCAP, int ARENA, int CMAX>
 struct Pass1Smem {
   static constexpr int ECAP = 2 * (CAP + CAP);
@@ -728,11 +742,26 @@ __global__ void gather_kernel(int64_t ncols, const T* src, const int64_t* start,
 // (VoronoiMorpho::negate, src/vor3d/Voronoi.cpp:18-55, negate_ray
 // MorphologyOperators.cpp:230-259).  Phase 0 writes lengths, phase 1 data.
 template <class T>
-__global__ void ring_complement_kernel(int nx, int ny, const int64_t* off, const T* z, T lo, T hi,
-                                       int phase, int64_t* out_len, const int64_t* out_off, T* out) {
-  const int64_t C = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int NX = nx + 2, NY = ny + 2;
-  if (C >= int64_t(NX) * NY) return;
+struct ColumnJob {  // one volume of a column-parallel kernel
+  int nx, ny;        // input grid
+  const int64_t* off;
+  const T* z;
+  T lo, hi;
+  int64_t* out_len;       // per output column
+  const int64_t* out_off;  // per output column (absolute into out)
+  T* out;
+};
+
+template <class T>
+__device__ void ring_column(const ColumnJob<T>& g, int64_t C, int phase) {
+  const int nx = g.nx, ny = g.ny;
+  const int64_t* off = g.off;
+  const T* z = g.z;
+  const T lo = g.lo, hi = g.hi;
+  int64_t* out_len = g.out_len;
+  const int64_t* out_off = g.out_off;
+  T* out = g.out;
+  const int NX = nx + 2;
   const int X = int(C % NX), Y = int(C / NX);
   int64_t s = 0, n = 0;
   if (X >= 1 && X <= nx && Y >= 1 && Y <= ny) {
@@ -760,13 +789,25 @@ __global__ void ring_complement_kernel(int nx, int ny, const int64_t* off, const
   if (!drop_last) out[w++] = hi;
 }
 
+template <class T>
+__global__ void ring_complement_kernel(ColumnJob<T> g, int phase) {
+  const int64_t C = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (C >= int64_t(g.nx + 2) * (g.ny + 2)) return;
+  ring_column(g, C, phase);
+}
+
 // Crop the ring and complement within [lo, hi] with parity counting
 // (negateInv, Voronoi.cpp:57-89; negate_ray_range, MorphologyOperators.cpp:282-315).
+// (here nx, ny are the output grid: the input is (nx + 2) x (ny + 2))
 template <class T>
-__global__ void crop_complement_kernel(int nx, int ny, const int64_t* off, const T* z, T lo, T hi,
-                                       int phase, int64_t* out_len, const int64_t* out_off, T* out) {
-  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (c >= int64_t(nx) * ny) return;
+__device__ void crop_column(const ColumnJob<T>& g, int64_t c, int phase) {
+  const int nx = g.nx;
+  const int64_t* off = g.off;
+  const T* z = g.z;
+  const T lo = g.lo, hi = g.hi;
+  int64_t* out_len = g.out_len;
+  const int64_t* out_off = g.out_off;
+  T* out = g.out;
   const int x = int(c % nx), y = int(c / nx);
   const int64_t C = int64_t(x + 1) + int64_t(nx + 2) * (y + 1);
   const int64_t s = off[C], n = off[C + 1] - s;
@@ -794,6 +835,57 @@ __global__ void crop_complement_kernel(int nx, int ny, const int64_t* off, const
   for (int64_t i = head; i < n - tail; ++i) out[w++] = z[s + i];
   if (add_hi) out[w++] = hi;
 }
+
+template <class T>
+__global__ void crop_complement_kernel(ColumnJob<T> g, int phase) {
+  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= int64_t(g.nx) * g.ny) return;
+  crop_column(g, c, phase);
+}
+
+// Batched forms: job j owns output columns [col_base[j], col_base[j + 1]).
+template <class T>
+__global__ void ring_complement_batch_kernel(const ColumnJob<T>* jobs, const int64_t* col_base, int njobs,
+                                             int phase) {
+  const int64_t C = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (C >= col_base[njobs]) return;
+  const int j = find_job(col_base, njobs, C);
+  ring_column(jobs[j], C - col_base[j], phase);
+}
+
+template <class T>
+__global__ void crop_complement_batch_kernel(const ColumnJob<T>* jobs, const int64_t* col_base, int njobs,
+                                             int phase) {
+  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= col_base[njobs]) return;
+  const int j = find_job(col_base, njobs, c);
+  crop_column(jobs[j], c - col_base[j], phase);
+}
+
+// Final offsets of a batch: job j's offsets off_j[0 .. ncols_j] (absolute
+// into its endpoint buffer) rebased to start at 0 into rel[out_base[j] ..];
+// first[j] = off_j[0] and count[j] = off_j[ncols_j] - off_j[0].
+struct FinalJob {
+  const int64_t* off;
+  int64_t ncols;
+  int64_t out_base;
+};
+
+__global__ void finalize_offsets_kernel(const FinalJob* jobs, const int64_t* entry_base, int njobs, int64_t* rel,
+                                        int64_t* first, int64_t* count) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= entry_base[njobs]) return;
+  const int j = find_job(entry_base, njobs, i);
+  const FinalJob g = jobs[j];
+  const int64_t c = i - entry_base[j];
+  const int64_t o0 = g.off[0];
+  rel[g.out_base + c] = g.off[c] - o0;
+  if (c == 0) {
+    first[j] = o0;
+    count[j] = g.off[g.ncols] - o0;
+  }
+}
+
 
 }  // namespace vo
 
@@ -831,6 +923,7 @@ struct Pass2GArgs {
   int* status;
   int ocap;        // intervals per output column list
   int table;       // nonzero: H comes from an LDS table of (rint + 1)^2 entries
+  int64_t o_base;  // first endpoint of this volume's tiles in o_z (batched launches)
 };
 
 // Dynamic LDS of the gather form, in this order: the interval lists (lo, hi;
@@ -881,16 +974,15 @@ __device__ __forceinline__ bool lane_unite(double* olo, double* ohi, int ocap, i
 
 extern __shared__ double vo_gather_lds[];
 
+// One tile = (x, y0 .. y0 + 63); consecutive tiles are consecutive x, whose
+// source windows overlap, so the launches hand them to one XCD (xcd_tile).
 template <class T>
-__global__ __launch_bounds__(kWave) void pass2_gather_kernel(Pass2GArgs<T> g) {
+__device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
   const int L = lane_id();
   const int ocap = g.ocap;
   double* olo = vo_gather_lds + L;
   double* ohi = vo_gather_lds + ocap * kWave + L;
   double* htab = vo_gather_lds + 2 * ocap * kWave;
-  // tile = (x, y0 .. y0 + 63); consecutive tiles are consecutive x, whose
-  // source windows overlap, so they meet in one XCD's L2
-  const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const int x = int(tile % g.nx);
   const int y0 = int(tile / g.nx) * kGatherTile;
   if (y0 >= g.ny) return;
@@ -1004,18 +1096,36 @@ __global__ __launch_bounds__(kWave) void pass2_gather_kernel(Pass2GArgs<T> g) {
     ohi[k * kWave] = chi;
   }
   if (ballot(active && !ok) != 0) {
-    if (L == 0) g.tile_list[1 + atomicAdd(g.tile_list, 1)] = int(tile);
+    if (!g.tile_list)
+      raise_status(g.status, kStatusGatherOverflow);  // batched: the host redoes this volume alone
+    else if (L == 0)
+      g.tile_list[1 + atomicAdd(g.tile_list, 1)] = int(tile);
     return;
   }
   if (!active) return;
   const int64_t col = int64_t(x) + int64_t(g.nx) * y;
-  const int64_t ob = tile * g.arena + int64_t(L) * 2 * ocap;
+  const int64_t ob = g.o_base + tile * g.arena + int64_t(L) * 2 * ocap;
   for (int i = 0; i < n; ++i) {
     g.o_z[ob + 2 * i] = T(olo[i * kWave]);
     g.o_z[ob + 2 * i + 1] = T(ohi[i * kWave]);
   }
   g.o_start[col] = ob;
   g.o_cnt[col] = 2 * n;
+}
+
+template <class T>
+__global__ __launch_bounds__(kWave) void pass2_gather_kernel(Pass2GArgs<T> g) {
+  gather_tile(g, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// Batched: job j owns the global tiles [tile_base[j], tile_base[j + 1]).
+template <class T>
+__global__ __launch_bounds__(kWave) void pass2_gather_batch_kernel(const Pass2GArgs<T>* jobs,
+                                                                   const int64_t* tile_base, int njobs) {
+  const int64_t t = xcd_tile(blockIdx.x, gridDim.x);
+  const int j = find_job(tile_base, njobs, t);
+  const Pass2GArgs<T> g = jobs[j];
+  gather_tile(g, t - tile_base[j]);
 }
 
 }  // namespace vo
@@ -1448,11 +1558,8 @@ struct ZState {
 };
 
 template <class T, int ECAP, int NB, int WMAX, int NW>
-__global__ __launch_bounds__(kWave) void pass1_zsweep_kernel(Pass1ZArgs<T> g) {
-  using S = Pass1ZSmem<T, ECAP, NB, WMAX>;
-  __shared__ S sm;
+__device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, ECAP, NB, WMAX>& sm) {
   const int L = lane_id();
-  const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const int x = int(tile % g.nx);
   const int y0 = int(tile / g.nx) * kZTile;
   if (y0 >= g.ny) return;
@@ -1656,6 +1763,23 @@ __global__ __launch_bounds__(kWave) void pass1_zsweep_kernel(Pass1ZArgs<T> g) {
   }
   if (ballot(active && !st.ok) != 0) raise_status(g.status, kStatusArenaOverflow);
   if (active) g.m_cnt[count_index(g.ny, x, y0 + L)] = st.count;
+}
+
+template <class T, int ECAP, int NB, int WMAX, int NW>
+__global__ __launch_bounds__(kWave) void pass1_zsweep_kernel(Pass1ZArgs<T> g) {
+  __shared__ Pass1ZSmem<T, ECAP, NB, WMAX> sm;
+  zsweep_tile<T, ECAP, NB, WMAX, NW>(g, xcd_tile(blockIdx.x, gridDim.x), sm);
+}
+
+// Batched: job j owns the global tiles [tile_base[j], tile_base[j + 1]).
+template <class T, int ECAP, int NB, int WMAX, int NW>
+__global__ __launch_bounds__(kWave) void pass1_zsweep_batch_kernel(const Pass1ZArgs<T>* jobs,
+                                                                   const int64_t* tile_base, int njobs) {
+  __shared__ Pass1ZSmem<T, ECAP, NB, WMAX> sm;
+  const int64_t t = xcd_tile(blockIdx.x, gridDim.x);
+  const int j = find_job(tile_base, njobs, t);
+  const Pass1ZArgs<T> g = jobs[j];
+  zsweep_tile<T, ECAP, NB, WMAX, NW>(g, t - tile_base[j], sm);
 }
 
 }  // namespace vo

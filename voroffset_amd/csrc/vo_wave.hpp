@@ -46,6 +46,7 @@ enum : int {
   kStatusP2Interiors = 8,     // pass 2: interior list
   kStatusP2Disks = 16,        // pass 2: disk list
   kStatusP2Slice = 32,        // pass 2: intervals of one line slice
+  kStatusGatherOverflow = 64, // pass 2 gather form, batched: a tile's lists outgrew LDS
   kStatusListOverflow = kStatusP1List | kStatusP1Union | kStatusP2Interiors | kStatusP2Disks | kStatusP2Slice,
 };
 
