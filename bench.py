@@ -97,6 +97,61 @@ def run_device(ctx, vol, radius, precision, steps, warmup):
     return dt / steps, p1 / steps, p2 / steps, count
 
 
+def batched_leg(n_volumes, steps, warmup, seed=4321):
+    """The uneven batch: n volumes, side log-uniform in [64, 512], Poisson(lambda
+    ~ U[2, 10]) segments per column, r ~ U[2, 32], ops uniform over dilation /
+    erosion / closing / opening.  Batched FP32 (one vo_morph_batch_device call
+    for all volumes) against plain per-volume FP64 calls, inputs resident in
+    HBM, each timed over whole batches."""
+    from voroffset_amd import _native, synth
+
+    specs = synth.batch_specs(n_volumes, seed=seed)
+    keep, jobs32, jobs64, cols = [], [], [], 0
+    for sp in specs:
+        v = synth.poisson_columns(sp["side"], sp["side"], lam=sp["lam"], zmax=1024, seed=sp["seed"])
+        zlo, zhi = v.z_range()
+        shift = float(np.round(0.5 * (zlo + zhi)))
+        d_off = _native.DeviceArray(v.off)
+        d32 = _native.DeviceArray((v.z - shift).astype(np.float32))
+        d64 = _native.DeviceArray(v.z)
+        keep += [d_off, d32, d64]
+        jobs32.append((sp["op"], sp["radius"], v.nx, v.ny, d_off.address, d32.address, zlo - shift, zhi - shift))
+        jobs64.append((sp["op"], sp["radius"], v.nx, v.ny, d_off.address, d64.address, zlo, zhi))
+        cols += v.ncols
+    ctx = _native.Context(0)
+    for _ in range(warmup):
+        ctx.morph_batch_device("fp32", jobs32)
+    _native.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        ctx.morph_batch_device("fp32", jobs32)
+    _native.synchronize()
+    tb = (time.perf_counter() - t) / steps
+    alone = ctx.last_batch_alone()
+
+    def plain():
+        for (op, r, nx, ny, off, z, zlo, zhi) in jobs64:
+            ctx.morph_device(op, "fp64", r, nx, ny, off, z, zlo, zhi)
+    for _ in range(warmup):
+        plain()
+    _native.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        plain()
+    _native.synchronize()
+    tp = (time.perf_counter() - t) / steps
+    ctx.close()
+    for d in keep:
+        d.free()
+    log(f"batched: {n_volumes} volumes, {cols} columns; fp32 batch {tb * 1e3:.2f} ms, plain fp64 {tp * 1e3:.2f} ms "
+        f"({tp / tb:.2f}x), {alone} volumes redone alone")
+    return {"workload": f"{n_volumes} volumes, side log-uniform [64, 512], Poisson(U[2,10]) segments/column, "
+                        f"r U[2,32], ops uniform over dilation/erosion/closing/opening, seed {seed}",
+            "columns": cols, "batched_fp32": {"value": round(cols / tb, 1), "ms_per_batch": round(tb * 1e3, 3)},
+            "plain_fp64": {"value": round(cols / tp, 1), "ms_per_batch": round(tp * 1e3, 3)},
+            "batched_over_plain": round(tp / tb, 3), "volumes_redone_alone": alone}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +164,7 @@ def main():
     ap.add_argument("--cpu-crop", type=int, default=512, help="crop side of the all-threads CPU sample")
     ap.add_argument("--cpu-crop1", type=int, default=192, help="crop side of the single-thread CPU sample")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 leg (counter runs)")
+    ap.add_argument("--batch-volumes", type=int, default=256, help="volumes of the batched leg (0: skip it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -132,6 +188,8 @@ def main():
         log(f"fp64: {f64_s * 1e3:.3f} ms/step, pass1 {f64_p1:.3f} ms, pass2 {f64_p2:.3f} ms")
         plain = {"value": round(ncols / f64_s, 1), "ms_per_step": round(f64_s * 1e3, 4),
                  "pass1_ms": round(f64_p1, 4), "pass2_ms": round(f64_p2, 4)}
+
+    batched = batched_leg(args.batch_volumes, max(2, args.steps // 5), 1) if args.batch_volumes > 0 else None
 
     m, cpu = (None, None)
     if not args.no_cpu:
@@ -173,7 +231,7 @@ def main():
                                f"Exp(40) gaps and lengths, dilation r={R:g}, one volume per launch",
                    "grid": [side, side], "radius": R, "storage": "fp32", "op": "dilation",
                    "pass1_ms": round(p1_ms, 4), "pass2_ms": round(p2_ms, 4),
-                   "plain_fp64": plain, "pass2_sweep_tiles": fallback,
+                   "plain_fp64": plain, "pass2_sweep_tiles": fallback, "batched": batched,
                    "s_in": round(s_in, 4), "s_out": round(s_out, 4), "m_oracle": m},
         "roofline": roofline,
         "cpu_baseline": cpu,

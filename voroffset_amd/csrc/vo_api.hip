@@ -102,7 +102,7 @@ struct vo_context {
   int64_t* host_words = nullptr;  // pinned: status + counts
   std::string error;
   int64_t last_outputs = 0;
-  int p1_caps = 96;         // pass-1 slots per column (grows on overflow)
+  int p1_caps = 96;         // pass-1 slots per column (grows on overflow; a multiple of vo::kSlotBlock)
   int p2_cap_per_col = 16;  // pass-2 arena endpoints per column (grows on overflow)
   bool force_sweep = false;  // VO_PASS2=sweep: always use the sweep form of pass 2
   int pass1_form = 0;        // VO_PASS1: 0 = z-sweep (default), 1 = "paint", 2 = "cull"

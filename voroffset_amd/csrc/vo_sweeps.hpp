@@ -56,12 +56,15 @@ struct Pass1Args {
   int* status;
 };
 
-// Slot layout of the pass-1 triples, y-minor: the j-th triples of the
-// columns (x, y .. y + 63) share cache lines.  Pass 1 computes 64 consecutive
-// y of one x-line per wavefront, so each wavefront fills whole lines itself;
-// pass 2 reads them back with the same lane-to-y mapping.
+// Slot layout of the pass-1 triples: blocks of kSlotBlock consecutive
+// triples of one column are contiguous, and the blocks of the columns
+// (x, y .. y + 63) follow each other (y-minor).  Pass 1 computes 64
+// consecutive y of one x-line per wavefront, so a wavefront fills whole cache
+// lines itself; pass 2 reads a block per lane with two 16-byte loads per
+// array, coalesced across the wavefront.  `caps` is a multiple of kSlotBlock.
+constexpr int kSlotBlock = 8;
 __device__ __forceinline__ int64_t slot_index(int ny, int caps, int x, int y, int j) {
-  return (int64_t(x) * caps + j) * ny + y;
+  return ((int64_t(x) * (caps / kSlotBlock) + j / kSlotBlock) * ny + y) * kSlotBlock + (j % kSlotBlock);
 }
 __device__ __forceinline__ int64_t count_index(int ny, int x, int y) { return int64_t(x) * ny + y; }
 
@@ -974,6 +977,21 @@ __device__ __forceinline__ bool lane_unite(double* olo, double* ohi, int ocap, i
 
 extern __shared__ double vo_gather_lds[];
 
+// 8 consecutive values from a 16-byte aligned address, as 16-byte loads.
+__device__ __forceinline__ void load_block(const float* p, float (&v)[kSlotBlock]) {
+  const float4 a = *reinterpret_cast<const float4*>(p), b = *reinterpret_cast<const float4*>(p + 4);
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+  v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+__device__ __forceinline__ void load_block(const double* p, double (&v)[kSlotBlock]) {
+#pragma unroll
+  for (int i = 0; i < kSlotBlock; i += 2) {
+    const double2 a = *reinterpret_cast<const double2*>(p + i);
+    v[i] = a.x;
+    v[i + 1] = a.y;
+  }
+}
+
 // One tile = (x, y0 .. y0 + 63); consecutive tiles are consecutive x, whose
 // source windows overlap, so the launches hand them to one XCD (xcd_tile).
 template <class T>
@@ -991,7 +1009,11 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
   const int y = y0 + L;
   const bool active = y < g.ny;
   const int c_lo = max(0, x - rint), c_hi = min(g.nx - 1, x + rint);
-  auto count_of = [&](int c) { return active ? min(g.m_cnt[count_index(g.ny, c, y)], g.caps) : 0; };
+  const int yc = active ? y : g.ny - 1;  // lanes past the grid read a valid row and ignore it
+  auto count_of = [&](int c) {
+    const int m = min(g.m_cnt[count_index(g.ny, c, yc)], g.caps);
+    return active ? m : 0;
+  };
   int jmax = 0;
   for (int c = c_lo; c <= c_hi; ++c) jmax = max(jmax, count_of(c));
 #pragma unroll
@@ -1015,13 +1037,7 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
   int k = 0;
   double clo = INFINITY, chi = -INFINITY, below = -INFINITY, above = INFINITY;
   bool dirty = false;
-  auto unite = [&](double s, double e) {
-    if (s <= chi && e >= clo && s > below && e < above) {
-      clo = fmin(clo, s);
-      chi = fmax(chi, e);
-      dirty = true;
-      return;
-    }
+  auto unite_slow = [&](double s, double e) {
     if (dirty) {
       olo[k * kWave] = clo;
       ohi[k * kWave] = chi;
@@ -1036,12 +1052,28 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
     below = k > 0 ? ohi[(k - 1) * kWave] : -INFINITY;
     above = k + 1 < n ? olo[(k + 1) * kWave] : INFINITY;
   };
+  // unites [s, e] where `want` holds; written with selects so that the
+  // wavefront branches only when some lane needs the LDS list
+  auto unite = [&](bool want, double s, double e) {
+    if (ballot(want) == 0) return;  // the usual case inside a run
+    const bool fast = s <= chi && e >= clo && s > below && e < above;
+    const bool f = want && fast;
+    clo = f ? fmin(clo, s) : clo;
+    chi = f ? fmax(chi, e) : chi;
+    dirty = dirty || f;
+    const bool slow = want && !fast;
+    if (ballot(slow) != 0) {
+      if (slow) unite_slow(s, e);
+    }
+  };
   // Blocks of B pieces (j) outermost, source columns nearest first within a
   // block (d = 0, -1, +1, -2, ...): a block touches a few cache lines per
   // source column, which the neighbouring tiles read at about the same time.
   // Within a source column the pieces are z-sorted, and touching dilated
-  // pieces fuse into a run in registers before they reach the list.
-  constexpr int B = 8;
+  // pieces fuse into a run in registers before they reach the list.  Loads
+  // are unconditional from clamped in-bounds slots (values past a column's
+  // count are read and discarded), so the inner loop needs no branches.
+  constexpr int B = kSlotBlock;
   for (int jb = 0; jb < jmax; jb += B) {
     for (int step = 0; step <= 2 * rint; ++step) {
       const int ad = (step + 1) >> 1;
@@ -1050,45 +1082,41 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
       if (c < 0 || c >= g.nx) continue;  // uniform
       const int m = count_of(c);
       if (ballot(m > jb && ok) == 0) continue;
+      // the lane's block: B contiguous triples (jb is a multiple of B and
+      // below caps, so the whole block is inside the column's slots)
+      const int64_t o = slot_index(g.ny, g.caps, c, yc, jb);
       T va[B], vb[B];
       int vdy[B];
+      load_block(g.m_lo + o, va);
+      load_block(g.m_hi + o, vb);
+      {
+        const uint4 q = *reinterpret_cast<const uint4*>(g.m_dy + o);
+        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
-      for (int u = 0; u < B; ++u) {
-        va[u] = vb[u] = T(0);
-        vdy[u] = 0;
-        if (jb + u < m) {
-          const int64_t o = slot_index(g.ny, g.caps, c, y, jb + u);
-          va[u] = g.m_lo[o];
-          vb[u] = g.m_hi[o];
-          vdy[u] = g.m_dy[o];
-        }
+        for (int u = 0; u < B; ++u) vdy[u] = int((w[u >> 1] >> (16 * (u & 1))) & 0xffffu);
       }
       bool run = false;
       double rs = 0, re = 0;
 #pragma unroll
       for (int u = 0; u < B; ++u) {
-        if (jb + u >= m || !ok) continue;
-        const int dy = vdy[u];
+        const int dy = min(vdy[u], rint);
         double h;
         if (table) {
-          h = dy <= rint ? htab[dy * (rint + 1) + ad] : -1.0;
+          h = htab[dy * (rint + 1) + ad];
         } else {
           const double v = g.R2 - double(dy) * double(dy) - double(ad) * double(ad);
           h = v >= 0 ? sqrt(v) : -1.0;
         }
-        if (h < 0) continue;
+        const bool valid = ok && jb + u < m && vdy[u] <= rint && h >= 0;
         const double s = double(va[u]) - h, e = double(vb[u]) + h;
-        if (run && s <= re) {  // pieces are z-sorted, so e >= rs: the run stays connected
-          rs = fmin(rs, s);
-          re = fmax(re, e);
-          continue;
-        }
-        if (run) unite(rs, re);
-        run = true;
-        rs = s;
-        re = e;
+        // pieces are z-sorted, so e >= rs whenever s <= re: the run stays connected
+        const bool ext = valid && run && s <= re;
+        unite(valid && run && !ext, rs, re);
+        rs = ext ? fmin(rs, s) : (valid ? s : rs);
+        re = ext ? fmax(re, e) : (valid ? e : re);
+        run = run || valid;
       }
-      if (run && ok) unite(rs, re);
+      unite(run && ok, rs, re);
     }
   }
   if (dirty && ok) {
@@ -1431,6 +1459,7 @@ struct Pass1ZArgs {
 };
 
 constexpr int kZTile = kWave;    // output columns per wavefront
+constexpr int kGroupEnd = 1 << 30;  // event flag: last of its (z, kind) group
 constexpr int kMaxSlabs = 64;    // z-slabs per window
 
 template <class T, int ECAP, int NB, int WMAX>
@@ -1471,6 +1500,8 @@ struct ZState {
   int dist = 1 << 20, dcur = 1 << 20, count = 0;
   double z0 = 0;
   bool point_open = false, ok = true;
+  int64_t slot = 0;        // slot of triple `count` of this lane's column
+  int64_t block_jump = 0;  // slot step from the last triple of a block to the next block
 
   __device__ __forceinline__ explicit ZState(int r) : R(r) {
 #pragma unroll
@@ -1521,38 +1552,50 @@ struct ZState {
     }
     return best;
   }
+  // The updates below are written with selects: every lane runs them for
+  // every event, and only the store of a finished piece is predicated.
   __device__ __forceinline__ void set(int k) {
-    if (unsigned(k) > unsigned(2 * R)) return;
-    flip(k, true);
-    const int dd = k > R ? k - R : R - k;
-    if (dd < dist) dist = dd;
+    const bool in = unsigned(k) <= unsigned(2 * R);
+    const int kk = in ? k : R;
+    const uint64_t b = in ? (uint64_t(1) << (kk & 63)) : 0;
+#pragma unroll
+    for (int t = 0; t < NW; ++t) w[t] |= (t == (kk >> 6)) ? b : 0;
+    const int dd = kk > R ? kk - R : R - kk;
+    dist = (in && dd < dist) ? dd : dist;
   }
   __device__ __forceinline__ void clear(int k) {
-    if (unsigned(k) > unsigned(2 * R)) return;
-    flip(k, false);
-    const int dd = k > R ? k - R : R - k;
-    if (dd == dist) dist = nearest();
+    const bool in = unsigned(k) <= unsigned(2 * R);
+    const int kk = in ? k : R;
+    const uint64_t b = in ? (uint64_t(1) << (kk & 63)) : 0;
+#pragma unroll
+    for (int t = 0; t < NW; ++t) w[t] &= (t == (kk >> 6)) ? ~b : ~uint64_t(0);
+    const int dd = kk > R ? kk - R : R - kk;
+    const bool again = in && dd == dist;
+    if constexpr (NW == 1) {
+      const int nd = nearest();  // a few instructions: cheaper than a branch
+      dist = again ? nd : dist;
+    } else {
+      if (again) dist = nearest();
+    }
   }
   // close the open piece and open the next one when the distance changed;
   // zero-length pieces are kept only when a point opened them
   template <class T>
-  __device__ __forceinline__ void emit(double z, bool is_point, bool active, const Pass1ZArgs<T>& g, int x, int y) {
-    if (dist == dcur) return;
-    if (dcur <= R && (z0 < z || point_open) && active) {
-      if (count < g.caps) {
-        const int64_t o = slot_index(g.ny, g.caps, x, y, count);
-        g.m_lo[o] = T(z0);
-        g.m_hi[o] = T(z);
-        g.m_dy[o] = uint16_t(dcur);
-      } else {
-        ok = false;
-      }
-      ++count;
+  __device__ __forceinline__ void emit(double z, bool is_point, bool active, const Pass1ZArgs<T>& g) {
+    const bool changed = dist != dcur;
+    const bool put = changed && dcur <= R && (z0 < z || point_open) && active;
+    const bool fits = count < g.caps;
+    if (put && fits) {
+      g.m_lo[slot] = T(z0);
+      g.m_hi[slot] = T(z);
+      g.m_dy[slot] = uint16_t(dcur);
     }
-    if (dist <= R) {
-      z0 = z;
-      point_open = is_point;
-    }
+    ok = ok && !(put && !fits);
+    count += put ? 1 : 0;
+    slot += put ? ((count % kSlotBlock == 0) ? block_jump : 1) : 0;
+    const bool open = changed && dist <= R;
+    z0 = open ? z : z0;
+    point_open = open ? is_point : point_open;
     dcur = dist;
   }
 };
@@ -1651,6 +1694,8 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
     return;
   }
   ZState<NW> st(R);
+  st.slot = slot_index(g.ny, g.caps, x, y0 + L, 0);
+  st.block_jump = int64_t(g.ny) * kSlotBlock - (kSlotBlock - 1);
   int b_lo = 0;
   for (int sl = 0; sl < nslab; ++sl) {
     const int b_hi = sm.slab_end[sl];
@@ -1721,6 +1766,14 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
       }
     }
     wave_sync();
+    // mark the last event of every group of equal (z, kind) (the slab's last
+    // event ends a group); the flag leaves the low bits alone, which is all a
+    // neighbouring lane reads here
+    for (int i = L; i < E; i += kWave) {
+      const bool end = i + 1 == E || sm.ez[i + 1] != sm.ez[i] || (sm.ep[i + 1] & 3) != (sm.ep[i] & 3);
+      if (end) sm.ep[i] |= kGroupEnd;
+    }
+    wave_sync();
     // sweep in (z, kind) order; a group of equal (z, kind) is applied whole
     // before the lanes compare distances.  Events are read 64 at a time into
     // registers and broadcast with readlane.
@@ -1729,31 +1782,20 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
       const int cnt = min(kWave, E - base);
       const T vz = L < cnt ? sm.ez[base + L] : T(0);
       const int vp = L < cnt ? sm.ep[base + L] : 0;
-      const bool more = base + cnt < E;
-      const T zn_blk = more ? sm.ez[base + cnt] : T(0);
-      const int kn_blk = more ? (sm.ep[base + cnt] & 3) : -1;
       for (int i = 0; i < cnt; ++i) {
-        const double z = double(lane_bcast(vz, i));
-        const int pk = __builtin_amdgcn_readlane(vp, i);
+        const int raw = __builtin_amdgcn_readlane(vp, i);
+        const int pk = raw & ~kGroupEnd;
         const int kind = pk & 3;
         if (kind == 2)
           st.clear((pk >> 2) - L);
         else
           st.set((pk >> 2) - L);
-        double zn = 0;
-        int kn = -1;
-        if (i + 1 < cnt) {
-          zn = double(lane_bcast(vz, i + 1));
-          kn = __builtin_amdgcn_readlane(vp, i + 1) & 3;
-        } else if (more) {
-          zn = double(zn_blk);
-          kn = kn_blk;
-        }
-        if (kn < 0 || zn != z || kn != kind) {
-          st.emit(z, kind == 1, active, g, x, y0 + L);
+        if (raw & kGroupEnd) {
+          const double z = double(lane_bcast(vz, i));
+          st.emit(z, kind == 1, active, g);
           if (kind == 1) {
-            for (int t = gstart; t <= base + i; ++t) st.clear((sm.ep[t] >> 2) - L);
-            st.emit(z, false, active, g, x, y0 + L);
+            for (int t = gstart; t <= base + i; ++t) st.clear(((sm.ep[t] & ~kGroupEnd) >> 2) - L);
+            st.emit(z, false, active, g);
           }
           gstart = base + i + 1;
         }
