@@ -98,6 +98,21 @@ int vo_morph_batch_device(vo_context* ctx, int precision, int n, const vo_job* j
  * (a tuning signal).  No reference counterpart. */
 int vo_last_batch_alone(const vo_context* ctx, int* jobs);
 
+/* Parity metrics on a device volume (z of the given precision): the measure
+ * spacing^3 * sum of segment lengths and the number of segments, as
+ * CompressedVolume::get_volume / numSegments (src/vor3d/CompressedVolume.cpp:61-83;
+ * an odd last endpoint of a column is ignored). */
+int vo_measure_device(vo_context* ctx, int precision, const vo_volume* v, double spacing, double* volume,
+                      int64_t* segments);
+
+/* VoronoiMorpho::calculateXor (src/vor3d/Voronoi.cpp:91-111): per column the
+ * symmetric difference of a and b (equal grids) within [a->z_lo, a->z_hi],
+ * composed as calculate_ray_xor does (MorphologyOperators.cpp:319-374,
+ * segments shorter than 1e-10 dropped).  `out` receives device pointers into
+ * the workspace (valid until the next call), `*volume` its measure. */
+int vo_xor_device(vo_context* ctx, int precision, const vo_volume* a, const vo_volume* b, double spacing,
+                  vo_volume* out, int64_t* out_count, double* volume);
+
 /* Mesh -> dexels: replaces the ray casting of vor3d::create_dexels
  * (src/vor3d/Dexelize.cpp:166-225; predicates :56-162).  `vertices` is
  * num_vertices x 3 doubles (x, y, z), `triangles` num_triangles x 3 vertex

@@ -52,6 +52,9 @@ def lib():
         L.vo_oracle_rasterize.argtypes = [_f64p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32), ctypes.c_int64,
                                           ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                           ctypes.c_double]
+        L.vo_oracle_xor.restype = ctypes.c_void_p
+        L.vo_oracle_xor.argtypes = [ctypes.c_int, ctypes.c_int, _i64p, _f64p, _i64p, _f64p, ctypes.c_double,
+                                    ctypes.c_double]
         L.vo_oracle_union.restype = ctypes.c_int
         L.vo_oracle_union.argtypes = [_f64p, ctypes.c_int, _f64p, ctypes.c_int, _f64p, ctypes.c_int]
         _LIB = L
@@ -109,6 +112,36 @@ def pass1(vol, radius, threads=1):
     finally:
         L.vo_oracle_free(h)
     return o, y1, y2, r
+
+
+def xor(a, b):
+    """calculateXor per column (Voronoi.cpp:91-111) within a's z-range: (off, z)."""
+    L = lib()
+    oa, za = _csr(a)
+    ob, zb = _csr(b)
+    lo, hi = a.z_range()
+    h = L.vo_oracle_xor(a.nx, a.ny, _p64(oa), _pf(za), _p64(ob), _pf(zb), lo, hi)
+    if not h:
+        raise RuntimeError(L.vo_oracle_last_error().decode())
+    try:
+        n = L.vo_oracle_count(h)
+        o = np.empty(a.ncols + 1, np.int64)
+        out = np.empty(n, np.float64)
+        L.vo_oracle_copy(h, _p64(o), _pf(out), None, None)
+    finally:
+        L.vo_oracle_free(h)
+    return o, out
+
+
+def measure(off, z, spacing):
+    """get_volume / numSegments (CompressedVolume.cpp:61-83) in numpy."""
+    lens = np.diff(off)
+    total, segs = 0.0, 0
+    for c in np.nonzero(lens >= 2)[0]:
+        col = z[off[c]:off[c] + (lens[c] // 2) * 2]
+        total += float(np.sum(col[1::2] - col[0::2]))
+        segs += int(lens[c] // 2)
+    return total * spacing ** 3, segs
 
 
 def rasterize(vertices, triangles, nx, ny, origin_x, origin_y, spacing):

@@ -19,6 +19,7 @@
 //   VoronoiBruteForce.cpp:16-100 closed-form dilation
 //   app/cli3d/offset3d.cpp:124-133  closing / opening composition
 //   Dexelize.cpp:56-225          mesh -> dexels (ray casting, SOS predicates)
+//   Voronoi.cpp:91-111, MorphologyOperators.cpp:319-374  calculateXor
 //
 // How each part is pinned (the reference itself cannot be compiled here: its
 // core includes <Eigen/Dense>, which this image does not have):
@@ -34,6 +35,8 @@
 //     The exact triples themselves are "parity unpinned".
 //   * 1-D merges and complements: textbook interval union / complement on
 //     hand-made cases and random property checks.
+//   * calculateXor: set identities (xor(A, A) empty, xor(A, empty) = A,
+//     measure = |A| + |B| - 2|A n B|) and pointwise membership on samples.
 //   * rasterizer: closed forms (axis-aligned boxes, dexel centres exactly on
 //     shared edges and vertices, where SOS must count a point exactly once),
 //     and brute force over all triangles (the tile pruning only skips
@@ -1015,6 +1018,30 @@ Result* to_result(const Volume& v) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Parity helpers (Voronoi.cpp:91-111, MorphologyOperators.cpp:319-374,
+// CompressedVolume.cpp:61-83).
+
+constexpr double kMinSegSize = 1e-10;  // MorphologyOperators.cpp:4-5
+
+// calculate_ray_xor: (A \ B) u (B \ A) through complements and unions,
+// then segments shorter than kMinSegSize dropped (removepoint).
+Column ray_xor(const Column& a, const Column& b, double lo, double hi) {
+  Column na = a, nb = b;
+  negate_ray(na, lo, hi);
+  negate_ray(nb, lo, hi);
+  Column t1 = union_columns(na, b), t2 = union_columns(nb, a);
+  negate_ray(t1, lo, hi);
+  negate_ray(t2, lo, hi);
+  Column u = union_columns(t1, t2), out;
+  for (size_t i = 0; i + 1 < u.size(); i += 2) {
+    if (u[i + 1] - u[i] < kMinSegSize) continue;
+    out.push_back(u[i]);
+    out.push_back(u[i + 1]);
+  }
+  return out;
+}
+
+// ---------------------------------------------------------------------------
 // Mesh -> dexels (Dexelize.cpp:56-225).  For each dexel centre
 // (origin_x + (x + 0.5) * spacing, origin_y + (y + 0.5) * spacing)
 // (CompressedVolumeBase.cpp:5-11), the z / spacing of every triangle whose
@@ -1161,6 +1188,22 @@ void* vo_oracle_pass1(int nx, int ny, const int64_t* off, const double* z, doubl
       }
     }
     return res;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return nullptr;
+  }
+}
+
+// calculateXor per column (the result volume; its measure is the caller's).
+void* vo_oracle_xor(int nx, int ny, const int64_t* off_a, const double* z_a, const int64_t* off_b,
+                    const double* z_b, double lo, double hi) {
+  try {
+    Volume a = from_csr(nx, ny, off_a, z_a), b = from_csr(nx, ny, off_b, z_b), r;
+    r.nx = nx;
+    r.ny = ny;
+    r.cols.resize(a.cols.size());
+    for (size_t c = 0; c < a.cols.size(); ++c) r.cols[c] = ray_xor(a.cols[c], b.cols[c], lo, hi);
+    return to_result(r);
   } catch (const std::exception& e) {
     g_error = e.what();
     return nullptr;

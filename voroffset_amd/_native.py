@@ -21,7 +21,7 @@ PRECISIONS = {"fp32": 0, "fp64": 1}
 # every symbol the header declares
 EXPORTS = ("vo_create", "vo_destroy", "vo_last_error", "vo_morph", "vo_result_shape", "vo_result_copy",
            "vo_result_free", "vo_morph_device", "vo_last_stats", "vo_rasterize", "vo_morph_batch_device",
-           "vo_last_batch_alone")
+           "vo_last_batch_alone", "vo_measure_device", "vo_xor_device")
 
 
 class VoError(RuntimeError):
@@ -75,6 +75,12 @@ def lib():
         L.vo_morph_batch_device.argtypes = [vp, i32, i32, ctypes.POINTER(VoJob), ctypes.POINTER(VoVolume),
                                             ctypes.POINTER(i64), ctypes.POINTER(dbl), ctypes.POINTER(dbl)]
         L.vo_morph_batch_device.restype = i32
+        L.vo_measure_device.argtypes = [vp, i32, ctypes.POINTER(VoVolume), dbl, ctypes.POINTER(dbl),
+                                        ctypes.POINTER(i64)]
+        L.vo_measure_device.restype = i32
+        L.vo_xor_device.argtypes = [vp, i32, ctypes.POINTER(VoVolume), ctypes.POINTER(VoVolume), dbl,
+                                    ctypes.POINTER(VoVolume), ctypes.POINTER(i64), ctypes.POINTER(dbl)]
+        L.vo_xor_device.restype = i32
         L.vo_last_batch_alone.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         L.vo_last_batch_alone.restype = i32
         _lib = L
@@ -177,6 +183,26 @@ class Context:
         finally:
             lib().vo_result_free(res)
         return o, zz
+
+    def measure_device(self, precision, nx, ny, off_ptr, z_ptr, spacing):
+        """(spacing^3 * total segment length, number of segments) of a device volume."""
+        v = VoVolume(nx, ny, off_ptr, z_ptr, 0.0, 0.0)
+        vol, segs = ctypes.c_double(0), ctypes.c_int64(0)
+        rc = lib().vo_measure_device(self._h, PRECISIONS[precision], ctypes.byref(v), float(spacing),
+                                     ctypes.byref(vol), ctypes.byref(segs))
+        if rc != OK:
+            self._raise(rc)
+        return vol.value, segs.value
+
+    def xor_device(self, precision, a, b, spacing):
+        """calculateXor of two device volumes given as VoVolume; returns
+        (VoVolume in the workspace, endpoint count, measure)."""
+        out, cnt, vol = VoVolume(), ctypes.c_int64(0), ctypes.c_double(0)
+        rc = lib().vo_xor_device(self._h, PRECISIONS[precision], ctypes.byref(a), ctypes.byref(b), float(spacing),
+                                 ctypes.byref(out), ctypes.byref(cnt), ctypes.byref(vol))
+        if rc != OK:
+            self._raise(rc)
+        return out, cnt.value, vol.value
 
     def last_batch_alone(self):
         """Jobs of the last batch that were redone by the single-volume path."""

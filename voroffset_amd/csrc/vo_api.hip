@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../../include/voroffset_hip.h"
+#include "vo_parity.hpp"
 #include "vo_raster.hpp"
 #include "vo_sweeps.hpp"
 
@@ -81,6 +82,7 @@ enum Slot {
   kIN_OFF, kIN_Z,                                                   // staged host input
   kR_V, kR_F, kR_TCNT, kR_TOFF, kR_TCUR, kR_TTRIS, kR_LEN, kR_OFF, kR_Z,  // rasterizer
   kB_TAB0, kB_TAB1, kB_TAB2, kB_STATUS, kB_FIN_OFF, kB_FIN_AUX,         // batched form
+  kX_SCR, kX_N1, kX_N2, kX_LEN, kX_OFF, kX_Z, kX_LENGTH, kX_SEGS, kX_SUM,  // parity metrics
   kB_OUT,                                  // + (stage * 3 + kind) * 2 (+1 for z): stage outputs
   kSLOTS = kB_OUT + 4 * 3 * 2
 };
@@ -699,6 +701,90 @@ void morph_batch_device(vo_context* ctx, int n, const vo_job* jobs, vo_volume* o
   }
 }
 
+// ---------------------------------------------------------------------------
+// Parity metrics (get_volume / numSegments, calculateXor) on device volumes.
+
+template <class T>
+void measure(vo_context* ctx, const vo_volume* v, double spacing, double* volume, int64_t* segments) {
+  validate_shape(ctx, v);
+  const int64_t n = int64_t(v->nx) * v->ny;
+  double vol = 0;
+  int64_t segs = 0;
+  if (n > 0) {
+    double* len = static_cast<double*>(ws(ctx, kX_LENGTH, size_t(n) * sizeof(double)));
+    int64_t* cnt = static_cast<int64_t*>(ws(ctx, kX_SEGS, size_t(n) * sizeof(int64_t)));
+    double* sum = static_cast<double*>(ws(ctx, kX_SUM, 2 * sizeof(double)));
+    hipLaunchKernelGGL(vo::measure_kernel<T>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, n, v->offsets,
+                       static_cast<const T*>(v->z), len, cnt);
+    check(ctx, hipGetLastError(), "measure launch");
+    size_t tmp = 0;
+    check(ctx, hipcub::DeviceReduce::Sum(nullptr, tmp, len, sum, int(n), ctx->stream), "reduce size");
+    void* t = ws(ctx, kSCAN_TMP, tmp);
+    check(ctx, hipcub::DeviceReduce::Sum(t, tmp, len, sum, int(n), ctx->stream), "reduce");
+    check(ctx, hipMemcpyAsync(&vol, sum, sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "copy");
+    check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    size_t tmp2 = 0;
+    int64_t* isum = reinterpret_cast<int64_t*>(sum + 1);
+    check(ctx, hipcub::DeviceReduce::Sum(nullptr, tmp2, cnt, isum, int(n), ctx->stream), "reduce size");
+    t = ws(ctx, kSCAN_TMP, tmp2);
+    check(ctx, hipcub::DeviceReduce::Sum(t, tmp2, cnt, isum, int(n), ctx->stream), "reduce");
+    segs = read_word(ctx, isum);
+  }
+  if (volume) *volume = vol * spacing * spacing * spacing;
+  if (segments) *segments = segs;
+}
+
+template <class T>
+void xor_device(vo_context* ctx, const vo_volume* a, const vo_volume* b, double spacing, vo_volume* out,
+                int64_t* out_count, double* volume) {
+  validate_shape(ctx, a);
+  validate_shape(ctx, b);
+  if (!out) fail(ctx, VO_ERR_ARGUMENT, "vo_xor_device: null output");
+  if (a->nx != b->nx || a->ny != b->ny) fail(ctx, VO_ERR_ARGUMENT, "calculateXor needs equal grids");
+  const int64_t n = int64_t(a->nx) * a->ny;
+  int64_t* off = static_cast<int64_t*>(ws(ctx, kX_OFF, size_t(n + 1) * sizeof(int64_t)));
+  T* z = nullptr;
+  int64_t total = 0;
+  if (n == 0) {
+    check(ctx, hipMemsetAsync(off, 0, sizeof(int64_t), ctx->stream), "memset");
+    z = static_cast<T*>(ws(ctx, kX_Z, 16));
+  } else {
+    // scratch: 2 * (endpoints of a + endpoints of b + 2 per column) doubles
+    int64_t ea[2], eb[2];
+    check(ctx, hipMemcpyAsync(&ea[0], a->offsets, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream), "copy");
+    check(ctx, hipMemcpyAsync(&ea[1], a->offsets + n, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream), "copy");
+    check(ctx, hipMemcpyAsync(&eb[0], b->offsets, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream), "copy");
+    check(ctx, hipMemcpyAsync(&eb[1], b->offsets + n, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream), "copy");
+    check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    const int64_t scr = 2 * ((ea[1] - ea[0]) + (eb[1] - eb[0]) + 2 * n);
+    double* scratch = static_cast<double*>(ws(ctx, kX_SCR, size_t(scr) * sizeof(double)));
+    int64_t* n1 = static_cast<int64_t*>(ws(ctx, kX_N1, size_t(n) * sizeof(int64_t)));
+    int64_t* n2 = static_cast<int64_t*>(ws(ctx, kX_N2, size_t(n) * sizeof(int64_t)));
+    int64_t* len = static_cast<int64_t*>(ws(ctx, kX_LEN, size_t(n + 1) * sizeof(int64_t)));
+    const double lo = a->z_lo, hi = a->z_hi;
+    const dim3 grid(blocks_for(n, 256));
+    hipLaunchKernelGGL(vo::xor_kernel<T>, grid, dim3(256), 0, ctx->stream, n, a->offsets, static_cast<const T*>(a->z),
+                       b->offsets, static_cast<const T*>(b->z), lo, hi, 0, scratch, n1, n2, len, nullptr, nullptr);
+    check(ctx, hipGetLastError(), "xor launch");
+    check(ctx, hipMemsetAsync(len + n, 0, sizeof(int64_t), ctx->stream), "memset");
+    exclusive_scan(ctx, len, off, n + 1);
+    total = read_word(ctx, off + n);
+    z = static_cast<T*>(ws(ctx, kX_Z, size_t(std::max<int64_t>(total, 1)) * sizeof(T)));
+    hipLaunchKernelGGL(vo::xor_kernel<T>, grid, dim3(256), 0, ctx->stream, n, a->offsets, static_cast<const T*>(a->z),
+                       b->offsets, static_cast<const T*>(b->z), lo, hi, 1, scratch, n1, n2, len, off, z);
+    check(ctx, hipGetLastError(), "xor launch");
+  }
+  out->nx = a->nx;
+  out->ny = a->ny;
+  out->offsets = off;
+  out->z = z;
+  out->z_lo = a->z_lo;
+  out->z_hi = a->z_hi;
+  if (out_count) *out_count = total;
+  if (volume) measure<T>(ctx, out, spacing, volume, nullptr);
+  check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+}
+
 int guarded(vo_context* ctx, const std::function<void()>& body) {
   if (!ctx) return VO_ERR_ARGUMENT;
   try {
@@ -979,6 +1065,30 @@ int vo_morph_batch_device(vo_context* ctx, int precision, int n, const vo_job* j
       morph_batch_device<float>(ctx, n, jobs, out, out_count, time_1_ms, time_2_ms);
     else if (precision == VO_FP64)
       morph_batch_device<double>(ctx, n, jobs, out, out_count, time_1_ms, time_2_ms);
+    else
+      fail(ctx, VO_ERR_ARGUMENT, "unknown precision");
+  });
+}
+
+int vo_measure_device(vo_context* ctx, int precision, const vo_volume* v, double spacing, double* volume,
+                      int64_t* segments) {
+  return guarded(ctx, [&] {
+    if (precision == VO_FP32)
+      measure<float>(ctx, v, spacing, volume, segments);
+    else if (precision == VO_FP64)
+      measure<double>(ctx, v, spacing, volume, segments);
+    else
+      fail(ctx, VO_ERR_ARGUMENT, "unknown precision");
+  });
+}
+
+int vo_xor_device(vo_context* ctx, int precision, const vo_volume* a, const vo_volume* b, double spacing,
+                  vo_volume* out, int64_t* out_count, double* volume) {
+  return guarded(ctx, [&] {
+    if (precision == VO_FP32)
+      xor_device<float>(ctx, a, b, spacing, out, out_count, volume);
+    else if (precision == VO_FP64)
+      xor_device<double>(ctx, a, b, spacing, out, out_count, volume);
     else
       fail(ctx, VO_ERR_ARGUMENT, "unknown precision");
   });

@@ -71,6 +71,44 @@ class VoronoiMorphoHip:
             for d in keep:
                 d.free()
 
+    def _upload(self, vol, shift=0.0):
+        import numpy as np
+        dtype = np.float32 if self.precision == "fp32" else np.float64
+        return _native.DeviceArray(vol.off), _native.DeviceArray((vol.z - shift).astype(dtype))
+
+    def get_volume(self, volume):
+        """(measure, number of segments): CompressedVolume::get_volume and
+        numSegments (CompressedVolume.cpp:61-83), reduced on the device."""
+        import numpy as np
+        zlo, zhi = volume.z_range()
+        shift = float(np.round(0.5 * (zlo + zhi))) if self.precision == "fp32" else 0.0  # lengths do not see it
+        d_off, d_z = self._upload(volume, shift)
+        try:
+            return self.ctx.measure_device(self.precision, volume.nx, volume.ny, d_off.address, d_z.address,
+                                           volume.spacing)
+        finally:
+            d_off.free()
+            d_z.free()
+
+    def calculate_xor(self, a, b):
+        """VoronoiMorpho::calculateXor (Voronoi.cpp:91-111): returns (measure
+        of the symmetric difference, the xor volume), with a's geometry."""
+        import numpy as np
+        dtype = np.float32 if self.precision == "fp32" else np.float64
+        zlo, zhi = a.z_range()
+        shift = float(np.round(0.5 * (zlo + zhi))) if self.precision == "fp32" else 0.0
+        arrays = [*self._upload(a, shift), *self._upload(b, shift)]
+        try:
+            va = _native.VoVolume(a.nx, a.ny, arrays[0].address, arrays[1].address, zlo - shift, zhi - shift)
+            vb = _native.VoVolume(b.nx, b.ny, arrays[2].address, arrays[3].address, zlo - shift, zhi - shift)
+            out, cnt, vol = self.ctx.xor_device(self.precision, va, vb, a.spacing)
+            off = _native.to_host(out.offsets, np.int64, a.nx * a.ny + 1)
+            z = _native.to_host(out.z, dtype, cnt).astype(np.float64) + shift
+            return vol, a.like(off, z)
+        finally:
+            for d in arrays:
+                d.free()
+
     def close(self):
         self.ctx.close()
 
