@@ -20,6 +20,7 @@ bounded crop of the same volume, on this host's cores.
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -152,6 +153,47 @@ def batched_leg(n_volumes, steps, warmup, seed=4321):
             "batched_over_plain": round(tp / tb, 3), "volumes_redone_alone": alone}
 
 
+def served_leg(rate, n_requests, pool, seed=99):
+    """Served form: `pool` resident volumes (side log-uniform in [128, 2048],
+    Poisson(8) segments per column), Poisson arrivals at `rate` per second of
+    n_requests mixed jobs (op uniform, r ~ U[2, 48]); FP32 storage, HBM-budget
+    admission, batches drained first-come first-served."""
+    from voroffset_amd import serve, synth
+
+    rng = np.random.default_rng(seed)
+    sides = np.round(np.exp(rng.uniform(math.log(128), math.log(2048), pool))).astype(int)
+    srv = serve.MorphoServer("fp32", hbm_budget=64 << 30, max_batch=64)
+    try:
+        handles, cols = [], {}
+        for i, side in enumerate(sides):
+            v = synth.poisson_columns(int(side), int(side), lam=8, zmax=1024, seed=seed * 1000 + i)
+            h = srv.put(v)
+            handles.append(h)
+            cols[h] = v.ncols
+        for h in handles:  # warm the workspace with the largest jobs
+            srv.submit(h, "closing", 48.0)
+        while srv.queue:
+            for req in srv.step():
+                srv.drop(req.result)
+        srv.batches.clear()
+        t = time.perf_counter()
+        done = serve.simulate(srv, handles, rate, n_requests, seed=seed)
+        span = max(r.done for r in done) - min(r.arrival for r in done)
+        rep = serve.latency_report(done)
+        total = sum(cols[r.handle] for r in done)
+        rep.update({"workload": f"{pool} resident volumes, side log-uniform [128, 2048], Poisson(8) segments/column; "
+                                f"{n_requests} requests, Poisson arrivals at {rate:g}/s, op uniform, r U[2,48]",
+                    "offered_rate_per_s": rate, "columns_per_s": round(total / span, 1),
+                    "requests_per_s": round(len(done) / span, 2), "batches": len(srv.batches),
+                    "mean_batch": round(float(np.mean([b for b, _ in srv.batches])), 2),
+                    "wall_s": round(time.perf_counter() - t, 3)})
+        log(f"served: p50 {rep['p50_ms']} ms, p99 {rep['p99_ms']} ms, {rep['requests_per_s']} req/s, "
+            f"{rep['columns_per_s']:.3g} columns/s, {rep['batches']} batches")
+        return rep
+    finally:
+        srv.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +207,9 @@ def main():
     ap.add_argument("--cpu-crop1", type=int, default=192, help="crop side of the single-thread CPU sample")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 leg (counter runs)")
     ap.add_argument("--batch-volumes", type=int, default=256, help="volumes of the batched leg (0: skip it)")
+    ap.add_argument("--serve-requests", type=int, default=200, help="requests of the served leg (0: skip it)")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="offered requests per second")
+    ap.add_argument("--serve-pool", type=int, default=16, help="resident volumes of the served leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -190,6 +235,8 @@ def main():
                  "pass1_ms": round(f64_p1, 4), "pass2_ms": round(f64_p2, 4)}
 
     batched = batched_leg(args.batch_volumes, max(2, args.steps // 5), 1) if args.batch_volumes > 0 else None
+
+    served = served_leg(args.serve_rate, args.serve_requests, args.serve_pool) if args.serve_requests > 0 else None
 
     m, cpu = (None, None)
     if not args.no_cpu:
@@ -231,7 +278,7 @@ def main():
                                f"Exp(40) gaps and lengths, dilation r={R:g}, one volume per launch",
                    "grid": [side, side], "radius": R, "storage": "fp32", "op": "dilation",
                    "pass1_ms": round(p1_ms, 4), "pass2_ms": round(p2_ms, 4),
-                   "plain_fp64": plain, "pass2_sweep_tiles": fallback, "batched": batched,
+                   "plain_fp64": plain, "pass2_sweep_tiles": fallback, "batched": batched, "served": served,
                    "s_in": round(s_in, 4), "s_out": round(s_out, 4), "m_oracle": m},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -288,6 +288,12 @@ class DeviceArray:
             pass
 
 
+def copy_device(dst, src, nbytes):
+    """Device-to-device copy (synchronous)."""
+    if nbytes:
+        _hip_check(hip().hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), int(nbytes), 3), "hipMemcpy D2D")
+
+
 def to_host(address, dtype, count):
     out = np.empty(count, dtype)
     if count:

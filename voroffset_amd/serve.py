@@ -1,0 +1,213 @@
+"""Served form: resident volumes, HBM-budget admission, batched execution.
+
+A ``MorphoServer`` keeps volumes resident in HBM (``put`` uploads once and
+returns a handle); requests name a handle, an operation and a radius.  Each
+``step`` admits waiting requests first-come first-served while the estimated
+device footprint of the batch stays within the HBM budget (and the batch
+within ``max_batch``), runs them as one ``vo_morph_batch_device`` call, and
+keeps every result resident as a new volume.  There is no reference
+counterpart: the reference runs one CLI invocation per volume
+(offset3d.cpp:116-136).
+"""
+
+from __future__ import annotations
+
+import collections
+import dataclasses
+import time
+
+import numpy as np
+
+from . import _native
+
+OPS = ("dilation", "erosion", "closing", "opening", "noop")
+_DILATIONS = {"noop": 0, "dilation": 1, "erosion": 1, "closing": 2, "opening": 2}
+
+
+@dataclasses.dataclass
+class Resident:
+    nx: int
+    ny: int
+    off: _native.DeviceArray
+    z: _native.DeviceArray
+    z_lo: float          # device z-range (shifted for FP32 storage)
+    z_hi: float
+    shift: float         # FP32 storage: device z = true z - shift
+    spacing: float
+    endpoints: int
+
+    @property
+    def ncols(self):
+        return self.nx * self.ny
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    handle: int
+    op: str
+    radius: float
+    arrival: float
+    start: float = 0.0
+    done: float = 0.0
+    result: int = -1     # handle of the resident result
+
+
+def footprint_bytes(ncols, endpoints, op, storage_bytes=4, caps=96, ocap=16):
+    """Estimated device bytes a job occupies inside a batch: the pass-1 slots
+    (lo, hi and a 16-bit line offset per slot), the pass-2 tile arenas, the
+    per-column counters and offsets, and the intermediate volumes of its
+    stages (complements grow the grid by a ring)."""
+    if op not in _DILATIONS:
+        raise ValueError(f"unknown operation {op!r}")
+    if _DILATIONS[op] == 0:
+        return 0
+    per_col = caps * (2 * storage_bytes + 2) + 2 * ocap * storage_bytes + 4 + 8 + 8 + 8
+    stages = 4 if op in ("closing", "opening") else (3 if op == "erosion" else 1)
+    volume = (ncols + 1) * 8 + (endpoints + 2 * ncols) * storage_bytes
+    return ncols * per_col + stages * volume
+
+
+class MorphoServer:
+    def __init__(self, precision="fp32", device=0, hbm_budget=64 << 30, max_batch=64, ctx=None):
+        if precision not in _native.PRECISIONS:
+            raise ValueError("precision must be 'fp32' or 'fp64'")
+        self.precision = precision
+        self.budget = int(hbm_budget)
+        self.max_batch = int(max_batch)
+        self.ctx = ctx if ctx is not None else _native.Context(device)
+        self.vols: dict[int, Resident] = {}
+        self.queue: collections.deque[Request] = collections.deque()
+        self._next = 0
+        self._rid = 0
+        self.batches = []    # (size, seconds) of every executed batch
+
+    # -- resident volumes ------------------------------------------------
+    def put(self, vol):
+        """Uploads a CompressedVolume; returns its handle."""
+        zlo, zhi = vol.z_range()
+        shift = float(np.round(0.5 * (zlo + zhi))) if self.precision == "fp32" else 0.0
+        dtype = np.float32 if self.precision == "fp32" else np.float64
+        r = Resident(vol.nx, vol.ny, _native.DeviceArray(vol.off), _native.DeviceArray((vol.z - shift).astype(dtype)),
+                     zlo - shift, zhi - shift, shift, vol.spacing, int(vol.z.size))
+        return self._register(r)
+
+    def _register(self, r):
+        h = self._next
+        self._next += 1
+        self.vols[h] = r
+        return h
+
+    def get(self, handle, like):
+        """Downloads a resident volume (geometry from `like`)."""
+        r = self.vols[handle]
+        dtype = np.float32 if self.precision == "fp32" else np.float64
+        off = _native.to_host(r.off.address, np.int64, r.ncols + 1)
+        z = _native.to_host(r.z.address, dtype, r.endpoints).astype(np.float64) + r.shift
+        return like.like(off, z)
+
+    def drop(self, handle):
+        r = self.vols.pop(handle)
+        r.off.free()
+        r.z.free()
+
+    # -- requests ----------------------------------------------------------
+    def submit(self, handle, op, radius, arrival=None):
+        if handle not in self.vols:
+            raise KeyError(f"no resident volume {handle}")
+        if op not in OPS:
+            raise ValueError(f"unknown operation {op!r}")
+        req = Request(self._rid, handle, op, float(radius), time.perf_counter() if arrival is None else arrival)
+        self._rid += 1
+        self.queue.append(req)
+        return req
+
+    def footprint(self, req):
+        r = self.vols[req.handle]
+        return footprint_bytes(r.ncols, r.endpoints, req.op, 4 if self.precision == "fp32" else 8)
+
+    def admit(self, now=None):
+        """Requests to run next: FIFO while the batch fits the budget (the
+        head of the queue always runs, alone if it exceeds the budget)."""
+        batch, used = [], 0
+        while self.queue and len(batch) < self.max_batch:
+            req = self.queue[0]
+            if now is not None and req.arrival > now:
+                break
+            need = self.footprint(req)
+            if batch and used + need > self.budget:
+                break
+            batch.append(self.queue.popleft())
+            used += need
+        return batch
+
+    def run(self, batch):
+        """Runs admitted requests as one batch; results stay resident."""
+        if not batch:
+            return []
+        jobs = []
+        for req in batch:
+            r = self.vols[req.handle]
+            jobs.append((req.op, req.radius, r.nx, r.ny, r.off.address, r.z.address, r.z_lo, r.z_hi))
+        t0 = time.perf_counter()
+        for req in batch:
+            req.start = t0
+        outs, _ = self.ctx.morph_batch_device(self.precision, jobs)
+        dtype = np.float32 if self.precision == "fp32" else np.float64
+        for req, (v, cnt) in zip(batch, outs):
+            src = self.vols[req.handle]
+            ncols = v.nx * v.ny
+            off = _native.DeviceArray(np.zeros(ncols + 1, np.int64))
+            z = _native.DeviceArray(np.zeros(max(cnt, 1), dtype))
+            _native.copy_device(off.address, v.offsets, (ncols + 1) * 8)
+            if cnt:
+                _native.copy_device(z.address, v.z, cnt * np.dtype(dtype).itemsize)
+            req.result = self._register(Resident(v.nx, v.ny, off, z, src.z_lo, src.z_hi, src.shift, src.spacing,
+                                                 int(cnt)))
+        _native.synchronize()
+        t1 = time.perf_counter()
+        for req in batch:
+            req.done = t1
+        self.batches.append((len(batch), t1 - t0))
+        return batch
+
+    def step(self, now=None):
+        return self.run(self.admit(now))
+
+    def close(self):
+        for h in list(self.vols):
+            self.drop(h)
+        self.ctx.close()
+
+
+def simulate(server, handles, rate, n_requests, radius_range=(2.0, 48.0), seed=0):
+    """Poisson arrivals at `rate` per second of n requests over the resident
+    `handles` (op uniform over the four operations, radius uniform); runs the
+    server in real time and returns the finished requests."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, n_requests)
+    t0 = time.perf_counter() + 0.01
+    arrivals = t0 + np.cumsum(gaps)
+    plan = [(int(rng.integers(len(handles))), OPS[int(rng.integers(4))], float(rng.uniform(*radius_range)))
+            for _ in range(n_requests)]
+    done, k = [], 0
+    while len(done) < n_requests:
+        now = time.perf_counter()
+        while k < n_requests and arrivals[k] <= now:
+            h, op, r = plan[k]
+            server.submit(handles[h], op, r, arrival=float(arrivals[k]))
+            k += 1
+        if not server.queue:
+            time.sleep(max(0.0, min(arrivals[k] - now, 0.01)))
+            continue
+        finished = server.step(now)
+        for req in finished:
+            server.drop(req.result)  # results are not reused here
+        done += finished
+    return done
+
+
+def latency_report(done):
+    lat = np.array([r.done - r.arrival for r in done]) * 1e3
+    return {"requests": len(done), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3), "mean_ms": round(float(lat.mean()), 3)}
