@@ -34,12 +34,13 @@ namespace {
 // Kernel shapes.  The LDS footprint of each instantiation is checked below.
 constexpr int kP1Cap = 128, kP1NCap = 64;
 constexpr int kP2SCap = 128, kP2PCap = 256, kP2NCap = 64, kP2OCap = 256;
-constexpr int kGHT = 40;  // gather form: H table in LDS up to this radius
 // gather form: intervals per output column list.  Large radii fuse the
 // dilated pieces into few intervals, so they get the short list (and more
 // waves per CU); a tile that overflows is recomputed by the sweep form.
 inline int gather_ocap(int rint) { return rint >= 6 ? 16 : 24; }
-constexpr int kZECap = 1536, kZNB = 512, kZWMax = 64 + 2 * 128;  // pass-1 z-sweep form
+// pass-1 z-sweep form: LDS events per z-slab (denser windows take more slabs)
+// and histogram bins; small enough for ~5 waves per SIMD
+constexpr int kZECap = 768, kZNB = 256, kZWMax = 64 + 2 * 128;
 
 template <class T>
 struct Shapes;
@@ -66,7 +67,6 @@ static_assert(sizeof(P1P<float>) <= 65536, "pass-1 painter LDS (fp32)");
 static_assert(sizeof(P1P<double>) <= 65536, "pass-1 painter LDS (fp64)");
 static_assert(sizeof(P2S<float>) <= 65536, "pass-2 LDS (fp32)");
 static_assert(sizeof(P2S<double>) <= 65536, "pass-2 LDS (fp64)");
-static_assert(vo::gather_lds_bytes(kGHT, 16, true) <= 65536, "pass-2 gather LDS");
 static_assert(sizeof(vo::Pass1ZSmem<double, kZECap, kZNB, kZWMax>) <= 65536, "pass-1 z-sweep LDS");
 
 struct Buffer {
@@ -82,6 +82,7 @@ enum Slot {
   kIN_OFF, kIN_Z,                                                   // staged host input
   kR_V, kR_F, kR_TCNT, kR_TOFF, kR_TCUR, kR_TTRIS, kR_LEN, kR_OFF, kR_Z,  // rasterizer
   kB_TAB0, kB_TAB1, kB_TAB2, kB_STATUS, kB_FIN_OFF, kB_FIN_AUX,         // batched form
+  kG_LISTS, kG_HTAB,                                         // gather interval lists, H table
   kX_SCR, kX_N1, kX_N2, kX_LEN, kX_OFF, kX_Z, kX_LENGTH, kX_SEGS, kX_SUM,  // parity metrics
   kB_OUT,                                  // + (stage * 3 + kind) * 2 (+1 for z): stage outputs
   kSLOTS = kB_OUT + 4 * 3 * 2
@@ -107,6 +108,7 @@ struct vo_context {
   int p1_caps = 96;         // pass-1 slots per column (grows on overflow; a multiple of vo::kSlotBlock)
   int p2_cap_per_col = 16;  // pass-2 arena endpoints per column (grows on overflow)
   bool force_sweep = false;  // VO_PASS2=sweep: always use the sweep form of pass 2
+  int zsweep_events = 768;   // VO_ZSWEEP_EVENTS: z-sweep LDS event capacity (512, 768, 1024, 1536)
   int pass1_form = 0;        // VO_PASS1: 0 = z-sweep (default), 1 = "paint", 2 = "cull"
   int last_fallback_tiles = 0;
   std::vector<void*> batch_extra;  // outputs of batched jobs redone alone (freed by the next batch)
@@ -186,6 +188,39 @@ std::string status_text(int s) {
   return "a sweep exceeded its LDS capacity:" + t;
 }
 
+// H(|d|, dy) = sqrt(R^2 - d^2 - dy^2) for the gather form, -1 where out of
+// reach; one (rint + 1)^2 table per radius, concatenated, uploaded to `slot`.
+// `host` keeps the values alive until the upload has run (the caller
+// synchronises before it goes out of scope).  Returns the device base.
+const double* upload_htab(vo_context* ctx, int slot, const std::vector<double>& radii, std::vector<double>& host) {
+  host.clear();
+  for (double R : radii) {
+    const int rint = int(std::floor(R));
+    const double R2 = R * R;
+    for (int ad = 0; ad <= rint; ++ad)
+      for (int dy = 0; dy <= rint; ++dy) {
+        const double v = R2 - double(dy) * double(dy) - double(ad) * double(ad);
+        host.push_back(v >= 0 ? std::sqrt(v) : -1.0);
+      }
+  }
+  double* d = static_cast<double*>(ws(ctx, slot, host.size() * sizeof(double)));
+  check(ctx, hipMemcpyAsync(d, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream),
+        "upload H table");
+  return d;
+}
+
+// The z-sweep form of pass 1 with 2R + 1 coverage bits per lane.
+template <class T, int ECAP, int NB>
+void launch_zsweep(vo_context* ctx, int rint, int64_t grid, const vo::Pass1ZArgs<T>& az) {
+  const dim3 g(static_cast<unsigned>(grid)), b(static_cast<unsigned>(vo::kWave));
+  if (rint <= 31)
+    hipLaunchKernelGGL((vo::pass1_zsweep_kernel<T, ECAP, NB, kZWMax, 1>), g, b, 0, ctx->stream, az);
+  else if (rint <= 63)
+    hipLaunchKernelGGL((vo::pass1_zsweep_kernel<T, ECAP, NB, kZWMax, 2>), g, b, 0, ctx->stream, az);
+  else
+    hipLaunchKernelGGL((vo::pass1_zsweep_kernel<T, ECAP, NB, kZWMax, 4>), g, b, 0, ctx->stream, az);
+}
+
 // One dilation: `in` -> CSR in (out_off_slot, out_z_slot).
 template <class T>
 DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int out_z_slot, double* t1,
@@ -210,6 +245,7 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
     out.z = ws(ctx, out_z_slot, 16);
     return out;
   }
+  std::vector<double> htab_host;  // outlives the uploads (read_word32 below synchronises)
   int chunk1 = std::min(Shapes<T>::p1_cmax, 32);
   int chunk2 = Shapes<T>::p2_cmax;
   bool zsweep_failed = false;  // a window too dense for the z-sweep form: sweep along y instead
@@ -227,7 +263,6 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
     const bool gather = rint <= 128 && !ctx->force_sweep;
     const int64_t ntiles = int64_t(nx) * ((ny + vo::kGatherTile - 1) / vo::kGatherTile);
     const int gocap = gather_ocap(rint);
-    const bool gtable = rint <= kGHT;
     const int64_t garena = gather ? int64_t(vo::kGatherTile) * 2 * gocap : 0;  // endpoints per gather tile
     const int chunk2_eff = gather ? 1 : chunk2;
     const int64_t arena2 = int64_t(chunk2_eff) * ctx->p2_cap_per_col;  // endpoints per sweep block
@@ -248,16 +283,12 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
       vo::Pass1ZArgs<T> az{nx, ny, rint, in.off, static_cast<const T*>(in.z), m_lo, m_hi, m_dy, caps, m_cnt,
                            status};
       const int64_t gridz = int64_t(nx) * ((ny + vo::kZTile - 1) / vo::kZTile);
-      // coverage mask words per lane: 2R + 1 bits
-      if (rint <= 31)
-        hipLaunchKernelGGL((vo::pass1_zsweep_kernel<T, kZECap, kZNB, kZWMax, 1>), dim3(unsigned(gridz)),
-                           dim3(vo::kWave), 0, ctx->stream, az);
-      else if (rint <= 63)
-        hipLaunchKernelGGL((vo::pass1_zsweep_kernel<T, kZECap, kZNB, kZWMax, 2>), dim3(unsigned(gridz)),
-                           dim3(vo::kWave), 0, ctx->stream, az);
-      else
-        hipLaunchKernelGGL((vo::pass1_zsweep_kernel<T, kZECap, kZNB, kZWMax, 4>), dim3(unsigned(gridz)),
-                           dim3(vo::kWave), 0, ctx->stream, az);
+      switch (ctx->zsweep_events) {
+        case 512: launch_zsweep<T, 512, 128>(ctx, rint, gridz, az); break;
+        case 1024: launch_zsweep<T, 1024, 256>(ctx, rint, gridz, az); break;
+        case 1536: launch_zsweep<T, 1536, 512>(ctx, rint, gridz, az); break;
+        default: launch_zsweep<T, kZECap, kZNB>(ctx, rint, gridz, az); break;
+      }
     } else if (ctx->pass1_form == 2) {
       hipLaunchKernelGGL((vo::pass1_kernel<T, kP1Cap, kP1NCap, Shapes<T>::p1_arena, Shapes<T>::p1_cmax>),
                          dim3(unsigned(grid1)), dim3(vo::kWave), 0, ctx->stream, a1);
@@ -269,11 +300,16 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
     check(ctx, hipEventRecord(ctx->ev[1], ctx->stream), "event");
     int nfall = 0;
     if (gather) {
+      double* lists = static_cast<double*>(ws(ctx, kG_LISTS, size_t(ntiles) * 2 * gocap * vo::kWave * sizeof(double)));
+      const double* ht = upload_htab(ctx, kG_HTAB, {R}, htab_host);
       vo::Pass2GArgs<T> ag{nx, ny, rint, R2, m_lo, m_hi, m_dy, caps, m_cnt, o_z, garena, o_start, o_len,
-                           tiles, status, gocap, gtable ? 1 : 0, 0};
-      const size_t lds = vo::gather_lds_bytes(rint, gocap, gtable);
-      hipLaunchKernelGGL((vo::pass2_gather_kernel<T>), dim3(unsigned(ntiles)), dim3(vo::kWave), lds,
-                         ctx->stream, ag);
+                           tiles, status, gocap, ht, 0, lists};
+      if (rint <= vo::kGatherLdsTableMax)
+        hipLaunchKernelGGL((vo::pass2_gather_kernel<T, true>), dim3(unsigned(ntiles)), dim3(vo::kWave),
+                           size_t(rint + 1) * (rint + 1) * sizeof(double), ctx->stream, ag);
+      else
+        hipLaunchKernelGGL((vo::pass2_gather_kernel<T, false>), dim3(unsigned(ntiles)), dim3(vo::kWave), 0,
+                           ctx->stream, ag);
       check(ctx, hipGetLastError(), "pass2 gather launch");
       nfall = int(read_word32(ctx, tiles));
     }
@@ -460,11 +496,15 @@ void dilate_batch(vo_context* ctx, std::vector<DevVol>& vol, const std::vector<d
       max_rint = std::max(max_rint, rint);
       ocap = std::max(ocap, gather_ocap(rint));
     }
-    size_t lds = 0;
+    std::vector<double> radii(K);
+    std::vector<int64_t> htab_base(K, 0);
     for (int k = 0; k < K; ++k) {
-      const int rint = int(std::floor(R[jobs[k]]));
-      lds = std::max(lds, vo::gather_lds_bytes(rint, ocap, rint <= kGHT));
+      radii[k] = R[jobs[k]];
+      const int64_t w = int64_t(std::floor(R[jobs[k]])) + 1;
+      if (k + 1 < K) htab_base[k + 1] = htab_base[k] + w * w;
     }
+    std::vector<double> htab_host;
+    const double* htab = upload_htab(ctx, kG_HTAB, radii, htab_host);
     const int64_t C = col_base[K], NT = tile_base[K];
     if (NT > INT32_MAX || C + 1 > INT32_MAX) fail(ctx, VO_ERR_ARGUMENT, "batch too large");
     const int64_t garena = int64_t(vo::kGatherTile) * 2 * ocap;
@@ -476,6 +516,8 @@ void dilate_batch(vo_context* ctx, std::vector<DevVol>& vol, const std::vector<d
     int64_t* o_start = static_cast<int64_t*>(ws(ctx, kO_START, size_t(C) * sizeof(int64_t)));
     int64_t* o_len = static_cast<int64_t*>(ws(ctx, kO_LEN, size_t(C + 1) * sizeof(int64_t)));
     int* status = static_cast<int*>(ws(ctx, kB_STATUS, size_t(K) * sizeof(int)));
+    const int64_t list_tile = 2 * int64_t(ocap) * vo::kWave;  // doubles of interval lists per tile
+    double* lists = static_cast<double*>(ws(ctx, kG_LISTS, size_t(NT) * list_tile * sizeof(double)));
     std::vector<vo::Pass1ZArgs<T>> a1(K);
     std::vector<vo::Pass2GArgs<T>> a2(K);
     for (int k = 0; k < K; ++k) {
@@ -486,7 +528,7 @@ void dilate_batch(vo_context* ctx, std::vector<DevVol>& vol, const std::vector<d
                                 caps, m_cnt + cb, status + k};
       a2[k] = vo::Pass2GArgs<T>{v.nx, v.ny, rint, R[jobs[k]] * R[jobs[k]], m_lo + sb, m_hi + sb, m_dy + sb, caps,
                                 m_cnt + cb, o_z, garena, o_start + cb, o_len + cb, nullptr, status + k, ocap,
-                                rint <= kGHT ? 1 : 0, tile_base[k] * garena};
+                                htab + htab_base[k], tile_base[k] * garena, lists + tile_base[k] * list_tile};
     }
     const auto* d1 = upload_table(ctx, kB_TAB0, a1);
     const auto* d2 = upload_table(ctx, kB_TAB1, a2);
@@ -509,7 +551,11 @@ void dilate_batch(vo_context* ctx, std::vector<DevVol>& vol, const std::vector<d
                          dt, K);
     check(ctx, hipGetLastError(), "batched pass1 launch");
     check(ctx, hipEventRecord(ctx->ev[1], ctx->stream), "event");
-    hipLaunchKernelGGL((vo::pass2_gather_batch_kernel<T>), grid, block, lds, ctx->stream, d2, dt, K);
+    if (max_rint <= vo::kGatherLdsTableMax)
+      hipLaunchKernelGGL((vo::pass2_gather_batch_kernel<T, true>), grid, block,
+                         size_t(max_rint + 1) * (max_rint + 1) * sizeof(double), ctx->stream, d2, dt, K);
+    else
+      hipLaunchKernelGGL((vo::pass2_gather_batch_kernel<T, false>), grid, block, 0, ctx->stream, d2, dt, K);
     check(ctx, hipGetLastError(), "batched pass2 launch");
     check(ctx, hipEventRecord(ctx->ev[2], ctx->stream), "event");
     // output CSR over all columns of the batch
@@ -943,6 +989,7 @@ int vo_create(vo_context** out, int device) {
   auto* ctx = new vo_context;
   ctx->device = device;
   if (const char* e = std::getenv("VO_PASS2")) ctx->force_sweep = std::string(e) == "sweep";
+  if (const char* e = std::getenv("VO_ZSWEEP_EVENTS")) ctx->zsweep_events = std::atoi(e);
   if (const char* e = std::getenv("VO_PASS1"))
     ctx->pass1_form = std::string(e) == "cull" ? 2 : (std::string(e) == "paint" ? 1 : 0);
   int rc = guarded(ctx, [&] {

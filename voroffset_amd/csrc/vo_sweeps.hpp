@@ -924,17 +924,12 @@ struct Pass2GArgs {
   int64_t* o_cnt;
   int* tile_list;  // [0] = count, then the ids of tiles to recompute with the sweep form
   int* status;
-  int ocap;        // intervals per output column list
-  int table;       // nonzero: H comes from an LDS table of (rint + 1)^2 entries
-  int64_t o_base;  // first endpoint of this volume's tiles in o_z (batched launches)
+  int ocap;             // intervals per output column list
+  const double* htab;   // H by (|d|, dy): htab[|d| * (rint + 1) + dy], -1 out of reach
+  int64_t o_base;       // first endpoint of this volume's tiles in o_z (batched launches)
+  double* lists;        // interval lists, 2 * ocap * 64 doubles per tile (lane-minor)
 };
 
-// Dynamic LDS of the gather form, in this order: the interval lists (lo, hi;
-// [ocap][64] doubles each, lane-minor so a wavefront's accesses are
-// conflict-free) and the H table.
-__host__ __device__ constexpr size_t gather_lds_bytes(int rint, int ocap, bool table) {
-  return sizeof(double) * (2 * size_t(ocap) * kWave + (table ? size_t(rint + 1) * (rint + 1) : 0));
-}
 
 // Unites [s, e] into the lane's sorted disjoint list (touching fuses),
 // starting the search at cursor k; on return k indexes the interval that now
@@ -975,8 +970,6 @@ __device__ __forceinline__ bool lane_unite(double* olo, double* ohi, int ocap, i
   return true;
 }
 
-extern __shared__ double vo_gather_lds[];
-
 // 8 consecutive values from a 16-byte aligned address, as 16-byte loads.
 __device__ __forceinline__ void load_block(const float* p, float (&v)[kSlotBlock]) {
   const float4 a = *reinterpret_cast<const float4*>(p), b = *reinterpret_cast<const float4*>(p + 4);
@@ -992,15 +985,23 @@ __device__ __forceinline__ void load_block(const double* p, double (&v)[kSlotBlo
   }
 }
 
+// Largest radius whose H table is staged in LDS ((R + 1)^2 doubles, 8 KB);
+// above it the lanes read the table from global memory (L1-resident).
+constexpr int kGatherLdsTableMax = 31;
+extern __shared__ double vo_htab_lds[];
+
 // One tile = (x, y0 .. y0 + 63); consecutive tiles are consecutive x, whose
 // source windows overlap, so the launches hand them to one XCD (xcd_tile).
-template <class T>
+template <class T, bool kLdsTable>
 __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
   const int L = lane_id();
   const int ocap = g.ocap;
-  double* olo = vo_gather_lds + L;
-  double* ohi = vo_gather_lds + ocap * kWave + L;
-  double* htab = vo_gather_lds + 2 * ocap * kWave;
+  // the lists are touched only when a run does not meet the cached interval,
+  // so they live in global memory (L1/L2-resident) and the kernel needs no
+  // LDS: occupancy is set by registers alone
+  double* const lbase = g.lists + tile * (2 * int64_t(ocap) * kWave);
+  double* olo = lbase + L;
+  double* ohi = lbase + ocap * kWave + L;
   const int x = int(tile % g.nx);
   const int y0 = int(tile / g.nx) * kGatherTile;
   if (y0 >= g.ny) return;
@@ -1008,6 +1009,12 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
   const int rint = g.rint;
   const int y = y0 + L;
   const bool active = y < g.ny;
+  const double* htab = g.htab;
+  if constexpr (kLdsTable) {
+    for (int i = L; i < (rint + 1) * (rint + 1); i += kWave) vo_htab_lds[i] = g.htab[i];
+    wave_sync();
+    htab = vo_htab_lds;
+  }
   const int c_lo = max(0, x - rint), c_hi = min(g.nx - 1, x + rint);
   const int yc = active ? y : g.ny - 1;  // lanes past the grid read a valid row and ignore it
   auto count_of = [&](int c) {
@@ -1018,16 +1025,6 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
   for (int c = c_lo; c <= c_hi; ++c) jmax = max(jmax, count_of(c));
 #pragma unroll
   for (int d = 1; d < kWave; d <<= 1) jmax = max(jmax, __shfl_xor(jmax, d));
-  const bool table = g.table != 0;
-  if (table) {
-    const int w1 = rint + 1;
-    for (int i = L; i < w1 * w1; i += kWave) {
-      const int dy = i / w1, ad = i % w1;
-      const double v = g.R2 - double(dy) * double(dy) - double(ad) * double(ad);
-      htab[i] = v >= 0 ? sqrt(v) : -1.0;
-    }
-  }
-  wave_sync();
   int n = 0;
   bool ok = true;
   // Interval k of the list is cached in registers as [clo, chi], with the
@@ -1082,6 +1079,7 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
       if (c < 0 || c >= g.nx) continue;  // uniform
       const int m = count_of(c);
       if (ballot(m > jb && ok) == 0) continue;
+      const double* hrow = htab + ad * (rint + 1);
       // the lane's block: B contiguous triples (jb is a multiple of B and
       // below caps, so the whole block is inside the column's slots)
       const int64_t o = slot_index(g.ny, g.caps, c, yc, jb);
@@ -1100,13 +1098,7 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
 #pragma unroll
       for (int u = 0; u < B; ++u) {
         const int dy = min(vdy[u], rint);
-        double h;
-        if (table) {
-          h = htab[dy * (rint + 1) + ad];
-        } else {
-          const double v = g.R2 - double(dy) * double(dy) - double(ad) * double(ad);
-          h = v >= 0 ? sqrt(v) : -1.0;
-        }
+        const double h = hrow[dy];
         const bool valid = ok && jb + u < m && vdy[u] <= rint && h >= 0;
         const double s = double(va[u]) - h, e = double(vb[u]) + h;
         // pieces are z-sorted, so e >= rs whenever s <= re: the run stays connected
@@ -1141,19 +1133,19 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
   g.o_cnt[col] = 2 * n;
 }
 
-template <class T>
+template <class T, bool kLdsTable>
 __global__ __launch_bounds__(kWave) void pass2_gather_kernel(Pass2GArgs<T> g) {
-  gather_tile(g, xcd_tile(blockIdx.x, gridDim.x));
+  gather_tile<T, kLdsTable>(g, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // Batched: job j owns the global tiles [tile_base[j], tile_base[j + 1]).
-template <class T>
+template <class T, bool kLdsTable>
 __global__ __launch_bounds__(kWave) void pass2_gather_batch_kernel(const Pass2GArgs<T>* jobs,
                                                                    const int64_t* tile_base, int njobs) {
   const int64_t t = xcd_tile(blockIdx.x, gridDim.x);
   const int j = find_job(tile_base, njobs, t);
   const Pass2GArgs<T> g = jobs[j];
-  gather_tile(g, t - tile_base[j]);
+  gather_tile<T, kLdsTable>(g, t - tile_base[j]);
 }
 
 }  // namespace vo
