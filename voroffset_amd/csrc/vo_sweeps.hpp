@@ -1093,6 +1093,18 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
 #pragma unroll
         for (int u = 0; u < B; ++u) vdy[u] = int((w[u >> 1] >> (16 * (u & 1))) & 0xffffu);
       }
+      // the block's triples lie in [a_first, b_last] (z-sorted); dilated by at
+      // most H(d, 0) they stay inside that range widened by it: when the
+      // cached interval holds it for every lane, the block changes nothing
+      {
+        const int nv = min(m - jb, B);  // valid triples of this lane's block (<= 0: none)
+        double blast = double(vb[0]);
+#pragma unroll
+        for (int u = 1; u < B; ++u) blast = u < nv ? double(vb[u]) : blast;
+        const double hmax = hrow[0];
+        const bool inside = nv <= 0 || !ok || (double(va[0]) - hmax >= clo && blast + hmax <= chi);
+        if (ballot(!inside) == 0) continue;
+      }
       bool run = false;
       double rs = 0, re = 0;
 #pragma unroll
@@ -1485,15 +1497,17 @@ __device__ __forceinline__ void column_segments(const T* z, int64_t s, int64_t n
 // reach (bit k <-> window column L + k, so the lane's own column is bit R),
 // held in NW 64-bit registers, the distance to the nearest covered column and
 // the open piece.
-template <int NW>
+// The sweep only compares endpoint values it read, so the open piece's start
+// is kept in the storage type T.
+template <int NW, class T>
 struct ZState {
   uint64_t w[NW];
   int R = 0;
   int dist = 1 << 20, dcur = 1 << 20, count = 0;
-  double z0 = 0;
+  T z0 = T(0);
   bool point_open = false, ok = true;
-  int64_t slot = 0;        // slot of triple `count` of this lane's column
-  int64_t block_jump = 0;  // slot step from the last triple of a block to the next block
+  int64_t slot = 0;    // slot of triple `count` of this lane's column
+  int block_jump = 0;  // slot step from the last triple of a block to the next block
 
   __device__ __forceinline__ explicit ZState(int r) : R(r) {
 #pragma unroll
@@ -1550,8 +1564,12 @@ struct ZState {
     const bool in = unsigned(k) <= unsigned(2 * R);
     const int kk = in ? k : R;
     const uint64_t b = in ? (uint64_t(1) << (kk & 63)) : 0;
+    if constexpr (NW == 1) {
+      w[0] |= b;  // 2R + 1 <= 64: every in-range bit is in word 0
+    } else {
 #pragma unroll
-    for (int t = 0; t < NW; ++t) w[t] |= (t == (kk >> 6)) ? b : 0;
+      for (int t = 0; t < NW; ++t) w[t] |= (t == (kk >> 6)) ? b : 0;
+    }
     const int dd = kk > R ? kk - R : R - kk;
     dist = (in && dd < dist) ? dd : dist;
   }
@@ -1559,8 +1577,12 @@ struct ZState {
     const bool in = unsigned(k) <= unsigned(2 * R);
     const int kk = in ? k : R;
     const uint64_t b = in ? (uint64_t(1) << (kk & 63)) : 0;
+    if constexpr (NW == 1) {
+      w[0] &= ~b;
+    } else {
 #pragma unroll
-    for (int t = 0; t < NW; ++t) w[t] &= (t == (kk >> 6)) ? ~b : ~uint64_t(0);
+      for (int t = 0; t < NW; ++t) w[t] &= (t == (kk >> 6)) ? ~b : ~uint64_t(0);
+    }
     const int dd = kk > R ? kk - R : R - kk;
     const bool again = in && dd == dist;
     if constexpr (NW == 1) {
@@ -1572,19 +1594,19 @@ struct ZState {
   }
   // close the open piece and open the next one when the distance changed;
   // zero-length pieces are kept only when a point opened them
-  template <class T>
-  __device__ __forceinline__ void emit(double z, bool is_point, bool active, const Pass1ZArgs<T>& g) {
+  __device__ __forceinline__ void emit(T z, bool is_point, bool active, const Pass1ZArgs<T>& g) {
     const bool changed = dist != dcur;
     const bool put = changed && dcur <= R && (z0 < z || point_open) && active;
     const bool fits = count < g.caps;
     if (put && fits) {
-      g.m_lo[slot] = T(z0);
-      g.m_hi[slot] = T(z);
+      g.m_lo[slot] = z0;
+      g.m_hi[slot] = z;
       g.m_dy[slot] = uint16_t(dcur);
     }
     ok = ok && !(put && !fits);
     count += put ? 1 : 0;
-    slot += put ? ((count % kSlotBlock == 0) ? block_jump : 1) : 0;
+    const int step = put ? ((count % kSlotBlock == 0) ? block_jump : 1) : 0;
+    slot += step;
     const bool open = changed && dist <= R;
     z0 = open ? z : z0;
     point_open = open ? is_point : point_open;
@@ -1685,9 +1707,9 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
     raise_status(g.status, kStatusP1List);
     return;
   }
-  ZState<NW> st(R);
+  ZState<NW, T> st(R);
   st.slot = slot_index(g.ny, g.caps, x, y0 + L, 0);
-  st.block_jump = int64_t(g.ny) * kSlotBlock - (kSlotBlock - 1);
+  st.block_jump = g.ny * kSlotBlock - (kSlotBlock - 1);
   int b_lo = 0;
   for (int sl = 0; sl < nslab; ++sl) {
     const int b_hi = sm.slab_end[sl];
@@ -1783,7 +1805,7 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
         else
           st.set((pk >> 2) - L);
         if (raw & kGroupEnd) {
-          const double z = double(lane_bcast(vz, i));
+          const T z = lane_bcast(vz, i);
           st.emit(z, kind == 1, active, g);
           if (kind == 1) {
             for (int t = gstart; t <= base + i; ++t) st.clear(((sm.ep[t] & ~kGroupEnd) >> 2) - L);
