@@ -32,6 +32,7 @@ extern "C" {
 
 typedef struct vo_context vo_context;
 typedef struct vo_result vo_result;
+typedef struct vo_mesh_result vo_mesh_result;
 
 enum vo_status {
   VO_OK = 0,
@@ -112,6 +113,18 @@ int vo_measure_device(vo_context* ctx, int precision, const vo_volume* v, double
  * the workspace (valid until the next call), `*volume` its measure. */
 int vo_xor_device(vo_context* ctx, int precision, const vo_volume* a, const vo_volume* b, double spacing,
                   vo_volume* out, int64_t* out_count, double* volume);
+
+/* Volume -> hexahedral boundary mesh: replaces dexel_dump
+ * (src/vor3d/Dexelize.cpp:312-351), computed on the device.  Host volume in
+ * (double z); every segment [z_i, z_{i+1}] of column (x, y) becomes the box
+ * [origin_x + x*s, origin_x + (x+1)*s] x [origin_y + y*s, ...] x [z_i*s, z_{i+1}*s]:
+ * 8 vertices in the reference's corner order and 6 outward quads (vertex
+ * indices), as a library-owned host result. */
+int vo_dexel_mesh(vo_context* ctx, const vo_volume* v, double origin_x, double origin_y, double spacing,
+                  vo_mesh_result** out);
+int vo_mesh_shape(const vo_mesh_result* r, int64_t* num_vertices, int64_t* num_quads);
+int vo_mesh_copy(const vo_mesh_result* r, double* vertices, int64_t* quads);
+void vo_mesh_free(vo_mesh_result* r);
 
 /* Mesh -> dexels: replaces the ray casting of vor3d::create_dexels
  * (src/vor3d/Dexelize.cpp:166-225; predicates :56-162).  `vertices` is

@@ -101,3 +101,39 @@ def test_hip_xor_of_morphology_against_oracle_is_empty(hip_fp64):
     ro, rz = oracle.morph(vol, "closing", 3.0)
     m, x = hip_fp64.calculate_xor(res, res.like(ro, rz))
     assert x.z.size == 0 and m == 0.0
+
+
+def _hex_reference(vol, ox, oy, s):
+    """dexel_dump's corners (Dexelize.cpp:318-340) in numpy: min + b * (max - min)."""
+    verts = []
+    for y in range(vol.ny):
+        for x in range(vol.nx):
+            col = vol.at(x, y)
+            for i in range(0, len(col) - 1, 2):
+                lo = np.array([ox + x * s, oy + y * s, col[i] * s])
+                hi = np.array([ox + (x + 1) * s, oy + (y + 1) * s, col[i + 1] * s])
+                for b in ((0, 0, 1), (1, 0, 1), (0, 1, 1), (1, 1, 1), (0, 0, 0), (1, 0, 0), (0, 1, 0), (1, 1, 0)):
+                    verts.append([lo[d] + b[d] * (hi[d] - lo[d]) for d in range(3)])
+    return np.array(verts).reshape(-1, 3)
+
+
+@pytest.mark.gpu
+def test_hip_dexel_mesh_matches_reference_formula():
+    from voroffset_amd import _native
+    ctx = _native.Context(0)
+    try:
+        vol = synth.poisson_columns(9, 7, lam=3, zmax=50, seed=8)
+        ox, oy, s = -1.3, 0.7, 0.37
+        vert, quad = ctx.dexel_mesh(vol.nx, vol.ny, vol.off, vol.z, ox, oy, s)
+        assert np.array_equal(vert, _hex_reference(vol, ox, oy, s))
+        nseg = vol.num_segments()
+        assert quad.shape == (6 * nseg, 4) and quad.max() == 8 * nseg - 1
+        # every quad is planar and its normal points away from its box's centre
+        v = vert[quad]
+        centre = vert.reshape(-1, 8, 3).mean(axis=1).repeat(6, axis=0)
+        normal = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
+        assert np.all(np.einsum("ij,ij->i", normal, v[:, 0] - centre) > 0)
+        empty_v, empty_q = ctx.dexel_mesh(0, 0, np.zeros(1, np.int64), np.zeros(0), 0.0, 0.0, 1.0)
+        assert empty_v.shape == (0, 3) and empty_q.shape == (0, 4)
+    finally:
+        ctx.close()

@@ -21,7 +21,8 @@ PRECISIONS = {"fp32": 0, "fp64": 1}
 # every symbol the header declares
 EXPORTS = ("vo_create", "vo_destroy", "vo_last_error", "vo_morph", "vo_result_shape", "vo_result_copy",
            "vo_result_free", "vo_morph_device", "vo_last_stats", "vo_rasterize", "vo_morph_batch_device",
-           "vo_last_batch_alone", "vo_measure_device", "vo_xor_device")
+           "vo_last_batch_alone", "vo_measure_device", "vo_xor_device",
+           "vo_dexel_mesh", "vo_mesh_shape", "vo_mesh_copy", "vo_mesh_free")
 
 
 class VoError(RuntimeError):
@@ -81,6 +82,14 @@ def lib():
         L.vo_xor_device.argtypes = [vp, i32, ctypes.POINTER(VoVolume), ctypes.POINTER(VoVolume), dbl,
                                     ctypes.POINTER(VoVolume), ctypes.POINTER(i64), ctypes.POINTER(dbl)]
         L.vo_xor_device.restype = i32
+        L.vo_dexel_mesh.argtypes = [vp, ctypes.POINTER(VoVolume), dbl, dbl, dbl, ctypes.POINTER(vp)]
+        L.vo_dexel_mesh.restype = i32
+        L.vo_mesh_shape.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+        L.vo_mesh_shape.restype = i32
+        L.vo_mesh_copy.argtypes = [vp, vp, vp]
+        L.vo_mesh_copy.restype = i32
+        L.vo_mesh_free.argtypes = [vp]
+        L.vo_mesh_free.restype = None
         L.vo_last_batch_alone.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         L.vo_last_batch_alone.restype = i32
         _lib = L
@@ -203,6 +212,26 @@ class Context:
         if rc != OK:
             self._raise(rc)
         return out, cnt.value, vol.value
+
+    def dexel_mesh(self, nx, ny, off, z, origin_x, origin_y, spacing):
+        """dexel_dump's hexahedra: (vertices (n, 3) float64, quads (m, 4) int64)."""
+        off = np.ascontiguousarray(off, np.int64)
+        z = np.ascontiguousarray(z, np.float64)
+        v = VoVolume(nx, ny, off.ctypes.data, z.ctypes.data if z.size else None, 0.0, 0.0)
+        res = ctypes.c_void_p()
+        rc = lib().vo_dexel_mesh(self._h, ctypes.byref(v), float(origin_x), float(origin_y), float(spacing),
+                                 ctypes.byref(res))
+        if rc != OK:
+            self._raise(rc)
+        try:
+            nv, nq = ctypes.c_int64(), ctypes.c_int64()
+            lib().vo_mesh_shape(res, ctypes.byref(nv), ctypes.byref(nq))
+            vert = np.empty((nv.value, 3), np.float64)
+            quad = np.empty((nq.value, 4), np.int64)
+            lib().vo_mesh_copy(res, vert.ctypes.data if vert.size else None, quad.ctypes.data if quad.size else None)
+        finally:
+            lib().vo_mesh_free(res)
+        return vert, quad
 
     def last_batch_alone(self):
         """Jobs of the last batch that were redone by the single-volume path."""

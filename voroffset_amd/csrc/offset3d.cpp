@@ -84,31 +84,25 @@ void save_volume(const std::string& path, const Volume& v) {
   }
 }
 
-void dump_obj(const std::string& path, const Volume& v) {
+// dexel_dump (Dexelize.cpp:312-351): the hexahedra come from the device
+// (vo_dexel_mesh); this writes them as OBJ quads.
+void dump_obj(vo_context* ctx, const std::string& path, const Volume& v) {
+  vo_volume hv{v.nx, v.ny, v.off.data(), v.z.empty() ? nullptr : v.z.data(), 0.0, 0.0};
+  vo_mesh_result* r = nullptr;
+  if (vo_dexel_mesh(ctx, &hv, v.origin[0], v.origin[1], v.spacing, &r) != VO_OK)
+    die(std::string("dexel mesh: ") + vo_last_error(ctx));
+  int64_t nv = 0, nq = 0;
+  vo_mesh_shape(r, &nv, &nq);
+  std::vector<double> vert(size_t(3 * nv));
+  std::vector<int64_t> quad(size_t(4 * nq));
+  vo_mesh_copy(r, vert.data(), quad.data());
+  vo_mesh_free(r);
   std::ofstream o(path);
   o << std::setprecision(17);
-  int64_t nvert = 0;
-  for (int y = 0; y < v.ny; ++y)
-    for (int x = 0; x < v.nx; ++x) {
-      const int64_t c = x + int64_t(v.nx) * y;
-      for (int64_t i = v.off[c]; i + 1 < v.off[c + 1]; i += 2) {
-        const double lo[3] = {v.origin[0] + x * v.spacing, v.origin[1] + y * v.spacing, v.z[i] * v.spacing};
-        const double hi[3] = {v.origin[0] + (x + 1) * v.spacing, v.origin[1] + (y + 1) * v.spacing,
-                              v.z[i + 1] * v.spacing};
-        // corners in dexel_dump's order: (0,0,1) (1,0,1) (0,1,1) (1,1,1) (0,0,0) (1,0,0) (0,1,0) (1,1,0)
-        static const int d[8][3] = {{0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1},
-                                    {0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0}};
-        for (const auto& k : d)
-          o << "v " << (k[0] ? hi[0] : lo[0]) << ' ' << (k[1] ? hi[1] : lo[1]) << ' ' << (k[2] ? hi[2] : lo[2])
-            << '\n';
-        static const int q[6][4] = {{4, 6, 7, 5}, {0, 1, 3, 2}, {4, 5, 1, 0},
-                                    {6, 2, 3, 7}, {4, 0, 2, 6}, {5, 7, 3, 1}};  // outward quads
-        for (const auto& f : q)
-          o << "f " << nvert + f[0] + 1 << ' ' << nvert + f[1] + 1 << ' ' << nvert + f[2] + 1 << ' '
-            << nvert + f[3] + 1 << '\n';
-        nvert += 8;
-      }
-    }
+  for (int64_t i = 0; i < nv; ++i) o << "v " << vert[3 * i] << ' ' << vert[3 * i + 1] << ' ' << vert[3 * i + 2] << '\n';
+  for (int64_t i = 0; i < nq; ++i)
+    o << "f " << quad[4 * i] + 1 << ' ' << quad[4 * i + 1] + 1 << ' ' << quad[4 * i + 2] + 1 << ' ' << quad[4 * i + 3] + 1
+      << '\n';
 }
 
 void dump_xyz(const std::string& path, const Volume& v) {
@@ -260,7 +254,7 @@ int main(int argc, char** argv) {
       std::cout << "Output mesh already exists. Please use -f to force overwriting." << std::endl;
     } else {
       const std::string e = vo_mesh::lower_ext(a.output_mesh);
-      if (e == ".obj") dump_obj(a.output_mesh, out);
+      if (e == ".obj") dump_obj(ctx, a.output_mesh, out);
       else if (e == ".xyz") dump_xyz(a.output_mesh, out);
       else save_volume(a.output_mesh, out);
     }

@@ -83,6 +83,7 @@ enum Slot {
   kR_V, kR_F, kR_TCNT, kR_TOFF, kR_TCUR, kR_TTRIS, kR_LEN, kR_OFF, kR_Z,  // rasterizer
   kB_TAB0, kB_TAB1, kB_TAB2, kB_STATUS, kB_FIN_OFF, kB_FIN_AUX,         // batched form
   kG_LISTS, kG_HTAB,                                         // gather interval lists, H table
+  kM_PAIRS, kM_SEGOFF, kM_VERT, kM_QUAD,                     // dexel mesh writer
   kX_SCR, kX_N1, kX_N2, kX_LEN, kX_OFF, kX_Z, kX_LENGTH, kX_SEGS, kX_SUM,  // parity metrics
   kB_OUT,                                  // + (stage * 3 + kind) * 2 (+1 for z): stage outputs
   kSLOTS = kB_OUT + 4 * 3 * 2
@@ -119,6 +120,11 @@ struct vo_result {
   int32_t nx = 0, ny = 0;
   std::vector<int64_t> off;
   std::vector<double> z;
+};
+
+struct vo_mesh_result {
+  std::vector<double> vertices;  // 3 per vertex
+  std::vector<int64_t> quads;    // 4 per quad
 };
 
 namespace {
@@ -831,6 +837,53 @@ void xor_device(vo_context* ctx, const vo_volume* a, const vo_volume* b, double 
   check(ctx, hipStreamSynchronize(ctx->stream), "sync");
 }
 
+// dexel_dump as a mesh, computed on the device from a host volume.
+void dexel_mesh(vo_context* ctx, const vo_volume* v, double ox, double oy, double spacing, vo_mesh_result** out) {
+  validate_shape(ctx, v);
+  if (!out || !(spacing > 0)) fail(ctx, VO_ERR_ARGUMENT, "vo_dexel_mesh: invalid arguments");
+  const int64_t n = int64_t(v->nx) * v->ny;
+  std::unique_ptr<vo_mesh_result> res(new vo_mesh_result);
+  const int64_t cnt = n ? v->offsets[n] - v->offsets[0] : 0;
+  if (n == 0 || cnt == 0) {
+    *out = res.release();
+    return;
+  }
+  int64_t* doff = static_cast<int64_t*>(ws(ctx, kIN_OFF, size_t(n + 1) * sizeof(int64_t)));
+  double* dz = static_cast<double*>(ws(ctx, kIN_Z, size_t(cnt) * sizeof(double)));
+  std::vector<int64_t> off0(v->offsets, v->offsets + n + 1);
+  for (auto& o : off0) o -= v->offsets[0];
+  check(ctx, hipMemcpyAsync(doff, off0.data(), size_t(n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream),
+        "upload");
+  check(ctx, hipMemcpyAsync(dz, static_cast<const double*>(v->z) + v->offsets[0], size_t(cnt) * sizeof(double),
+                            hipMemcpyHostToDevice, ctx->stream),
+        "upload");
+  int64_t* pairs = static_cast<int64_t*>(ws(ctx, kM_PAIRS, size_t(n + 1) * sizeof(int64_t)));
+  int64_t* seg_off = static_cast<int64_t*>(ws(ctx, kM_SEGOFF, size_t(n + 1) * sizeof(int64_t)));
+  const dim3 grid(blocks_for(n, 256));
+  hipLaunchKernelGGL(vo::pair_count_kernel, grid, dim3(256), 0, ctx->stream, n, doff, pairs);
+  check(ctx, hipGetLastError(), "pair count launch");
+  check(ctx, hipMemsetAsync(pairs + n, 0, sizeof(int64_t), ctx->stream), "memset");
+  exclusive_scan(ctx, pairs, seg_off, n + 1);
+  const int64_t segs = read_word(ctx, seg_off + n);
+  double* verts = static_cast<double*>(ws(ctx, kM_VERT, size_t(std::max<int64_t>(segs, 1)) * 24 * sizeof(double)));
+  int64_t* quads = static_cast<int64_t*>(ws(ctx, kM_QUAD, size_t(std::max<int64_t>(segs, 1)) * 24 * sizeof(int64_t)));
+  hipLaunchKernelGGL(vo::dexel_mesh_kernel<double>, grid, dim3(256), 0, ctx->stream, v->nx, v->ny, doff,
+                     static_cast<const double*>(dz), ox, oy, 0.0, spacing, seg_off, verts, quads);
+  check(ctx, hipGetLastError(), "dexel mesh launch");
+  res->vertices.resize(size_t(segs) * 24);
+  res->quads.resize(size_t(segs) * 24);
+  if (segs) {
+    check(ctx, hipMemcpyAsync(res->vertices.data(), verts, res->vertices.size() * sizeof(double),
+                              hipMemcpyDeviceToHost, ctx->stream),
+          "download");
+    check(ctx, hipMemcpyAsync(res->quads.data(), quads, res->quads.size() * sizeof(int64_t), hipMemcpyDeviceToHost,
+                              ctx->stream),
+          "download");
+  }
+  check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+  *out = res.release();
+}
+
 int guarded(vo_context* ctx, const std::function<void()>& body) {
   if (!ctx) return VO_ERR_ARGUMENT;
   try {
@@ -1140,6 +1193,34 @@ int vo_xor_device(vo_context* ctx, int precision, const vo_volume* a, const vo_v
       fail(ctx, VO_ERR_ARGUMENT, "unknown precision");
   });
 }
+
+int vo_dexel_mesh(vo_context* ctx, const vo_volume* v, double origin_x, double origin_y, double spacing,
+                  vo_mesh_result** out) {
+  vo_mesh_result* r = nullptr;
+  const int rc = guarded(ctx, [&] { dexel_mesh(ctx, v, origin_x, origin_y, spacing, &r); });
+  if (rc != VO_OK) {
+    delete r;
+    return rc;
+  }
+  *out = r;
+  return VO_OK;
+}
+
+int vo_mesh_shape(const vo_mesh_result* r, int64_t* num_vertices, int64_t* num_quads) {
+  if (!r) return VO_ERR_ARGUMENT;
+  if (num_vertices) *num_vertices = int64_t(r->vertices.size() / 3);
+  if (num_quads) *num_quads = int64_t(r->quads.size() / 4);
+  return VO_OK;
+}
+
+int vo_mesh_copy(const vo_mesh_result* r, double* vertices, int64_t* quads) {
+  if (!r) return VO_ERR_ARGUMENT;
+  if (vertices && !r->vertices.empty()) std::memcpy(vertices, r->vertices.data(), r->vertices.size() * sizeof(double));
+  if (quads && !r->quads.empty()) std::memcpy(quads, r->quads.data(), r->quads.size() * sizeof(int64_t));
+  return VO_OK;
+}
+
+void vo_mesh_free(vo_mesh_result* r) { delete r; }
 
 int vo_last_batch_alone(const vo_context* ctx, int* jobs) {
   if (!ctx || !jobs) return VO_ERR_ARGUMENT;

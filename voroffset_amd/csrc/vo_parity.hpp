@@ -183,4 +183,44 @@ __global__ void xor_kernel(int64_t ncols, const int64_t* off_a, const T* za, con
   if (phase == 0) len[c] = f.pos;
 }
 
+// dexel_dump (Dexelize.cpp:312-351): one hexahedron per segment of column
+// (x, y): min = (origin + (x, y) * s, z_i * s), max = (origin + (x + 1,
+// y + 1) * s, z_{i+1} * s), corners min + b * (max - min) in the reference's
+// corner order (0,0,1)
+// (1,0,1) (0,1,1) (1,1,1) (0,0,0) (1,0,0) (0,1,0) (1,1,0), and its six faces
+// as outward quads.  seg_off[c] = segments before column c.
+__constant__ int kHexQuads[6][4] = {{4, 6, 7, 5}, {0, 1, 3, 2}, {4, 5, 1, 0}, {6, 2, 3, 7}, {4, 0, 2, 6}, {5, 7, 3, 1}};
+
+template <class T>
+__global__ void dexel_mesh_kernel(int nx, int ny, const int64_t* off, const T* z, double ox, double oy,
+                                  double z_shift, double s, const int64_t* seg_off, double* vertices,
+                                  int64_t* quads) {
+#pragma clang fp contract(off)
+  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= int64_t(nx) * ny) return;
+  const int x = int(c % nx), y = int(c / nx);
+  const double x0 = ox + x * s, x1 = ox + (x + 1) * s;
+  const double y0 = oy + y * s, y1 = oy + (y + 1) * s;
+  int64_t seg = seg_off[c];
+  for (int64_t i = off[c]; i + 1 < off[c + 1]; i += 2, ++seg) {
+    const double z0 = (double(z[i]) + z_shift) * s, z1 = (double(z[i + 1]) + z_shift) * s;
+    // corner = min + b * (max - min), b in {0, 1}, as the reference forms it
+    const double xe = x0 + (x1 - x0), ye = y0 + (y1 - y0), ze = z0 + (z1 - z0);
+    double* v = vertices + seg * 24;
+    for (int k = 0; k < 8; ++k) {
+      v[3 * k] = (k & 1) ? xe : x0;
+      v[3 * k + 1] = (k & 2) ? ye : y0;
+      v[3 * k + 2] = (k < 4) ? ze : z0;
+    }
+    int64_t* q = quads + seg * 24;
+    for (int f = 0; f < 6; ++f)
+      for (int k = 0; k < 4; ++k) q[4 * f + k] = seg * 8 + kHexQuads[f][k];
+  }
+}
+
+__global__ void pair_count_kernel(int64_t ncols, const int64_t* off, int64_t* pairs) {
+  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c < ncols) pairs[c] = (off[c + 1] - off[c]) / 2;
+}
+
 }  // namespace vo
