@@ -73,6 +73,11 @@ def cpu_baseline(vol, radius, threads, crop, crop1):
                "host_cpus": os.cpu_count()}
 
 
+def _bucket(radius):
+    r = int(radius)
+    return 0 if r < 8 else 1 if r < 16 else 2 if r < 32 else 3 if r < 64 else 4
+
+
 def run_device(ctx, vol, radius, precision, steps, warmup):
     from voroffset_amd import _native
 
@@ -98,7 +103,23 @@ def run_device(ctx, vol, radius, precision, steps, warmup):
     return dt / steps, p1 / steps, p2 / steps, count
 
 
-def batched_leg(n_volumes, steps, warmup, seed=4321):
+def tune_context(ctx, lam, radii, side=256, seed=7):
+    """Untimed: autotunes ctx's shapes for these radii on a side x side
+    Poisson(lam) sample volume (FP32 storage)."""
+    from voroffset_amd import _native, synth
+
+    v = synth.poisson_columns(side, side, lam=lam, zmax=1024, seed=seed)
+    zlo, zhi = v.z_range()
+    shift = float(np.round(0.5 * (zlo + zhi)))
+    d_off, d_z = _native.DeviceArray(v.off), _native.DeviceArray((v.z - shift).astype(np.float32))
+    try:
+        return ctx.autotune("fp32", v.nx, v.ny, d_off.address, d_z.address, radii, reps=3)
+    finally:
+        d_off.free()
+        d_z.free()
+
+
+def batched_leg(n_volumes, steps, warmup, seed=4321, tune=True):
     """The uneven batch: n volumes, side log-uniform in [64, 512], Poisson(lambda
     ~ U[2, 10]) segments per column, r ~ U[2, 32], ops uniform over dilation /
     erosion / closing / opening.  Batched FP32 (one vo_morph_batch_device call
@@ -120,6 +141,8 @@ def batched_leg(n_volumes, steps, warmup, seed=4321):
         jobs64.append((sp["op"], sp["radius"], v.nx, v.ny, d_off.address, d64.address, zlo, zhi))
         cols += v.ncols
     ctx = _native.Context(0)
+    if tune:
+        tune_context(ctx, 6, [4.0, 12.0, 24.0, 32.0])
     for _ in range(warmup):
         ctx.morph_batch_device("fp32", jobs32)
     _native.synchronize()
@@ -153,7 +176,7 @@ def batched_leg(n_volumes, steps, warmup, seed=4321):
             "batched_over_plain": round(tp / tb, 3), "volumes_redone_alone": alone}
 
 
-def served_leg(rate, n_requests, pool, seed=99):
+def served_leg(rate, n_requests, pool, seed=99, tune=True):
     """Served form: `pool` resident volumes (side log-uniform in [128, 2048],
     Poisson(8) segments per column), Poisson arrivals at `rate` per second of
     n_requests mixed jobs (op uniform, r ~ U[2, 48]); FP32 storage, HBM-budget
@@ -164,6 +187,8 @@ def served_leg(rate, n_requests, pool, seed=99):
     sides = np.round(np.exp(rng.uniform(math.log(128), math.log(2048), pool))).astype(int)
     srv = serve.MorphoServer("fp32", hbm_budget=64 << 30, max_batch=64)
     try:
+        if tune:
+            tune_context(srv.ctx, 8, [4.0, 12.0, 24.0, 40.0])
         handles, cols = [], {}
         for i, side in enumerate(sides):
             v = synth.poisson_columns(int(side), int(side), lam=8, zmax=1024, seed=seed * 1000 + i)
@@ -206,6 +231,7 @@ def main():
     ap.add_argument("--cpu-crop", type=int, default=512, help="crop side of the all-threads CPU sample")
     ap.add_argument("--cpu-crop1", type=int, default=192, help="crop side of the single-thread CPU sample")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 leg (counter runs)")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default shapes (skip the autotuner)")
     ap.add_argument("--batch-volumes", type=int, default=256, help="volumes of the batched leg (0: skip it)")
     ap.add_argument("--serve-requests", type=int, default=200, help="requests of the served leg (0: skip it)")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="offered requests per second")
@@ -222,6 +248,17 @@ def main():
     vol = synth.poisson_columns(side, side, lam=6, zmax=1024, seed=1234)
     ncols = vol.ncols
     ctx = _native.Context(0)
+    tuned = None
+    if not args.no_tune:  # untimed: the autotuner picks the shapes for this radius on the workload itself
+        zlo, zhi = vol.z_range()
+        shift = float(np.round(0.5 * (zlo + zhi)))
+        d_off, d_z = _native.DeviceArray(vol.off), _native.DeviceArray((vol.z - shift).astype(np.float32))
+        t = time.perf_counter()
+        table = ctx.autotune("fp32", vol.nx, vol.ny, d_off.address, d_z.address, [R], reps=3)
+        d_off.free()
+        d_z.free()
+        tuned = {"events_per_slab": table[_bucket(R)][0], "gather_list_capacity": table[_bucket(R)][1]}
+        log(f"autotune ({time.perf_counter() - t:.1f} s): {tuned}")
 
     step_s, p1_ms, p2_ms, count = run_device(ctx, vol, R, "fp32", args.steps, args.warmup)
     fallback = ctx.last_stats()[1]
@@ -234,9 +271,12 @@ def main():
         plain = {"value": round(ncols / f64_s, 1), "ms_per_step": round(f64_s * 1e3, 4),
                  "pass1_ms": round(f64_p1, 4), "pass2_ms": round(f64_p2, 4)}
 
-    batched = batched_leg(args.batch_volumes, max(2, args.steps // 5), 1) if args.batch_volumes > 0 else None
+    # the mixed legs keep the default shapes: tuned on one sample they ran
+    # slower on the uneven volumes (measured: batch 110 ms tuned, 89 ms not)
+    batched = batched_leg(args.batch_volumes, max(2, args.steps // 5), 1, tune=False) if args.batch_volumes > 0 else None
 
-    served = served_leg(args.serve_rate, args.serve_requests, args.serve_pool) if args.serve_requests > 0 else None
+    served = (served_leg(args.serve_rate, args.serve_requests, args.serve_pool, tune=False)
+              if args.serve_requests > 0 else None)
 
     m, cpu = (None, None)
     if not args.no_cpu:
@@ -278,7 +318,8 @@ def main():
                                f"Exp(40) gaps and lengths, dilation r={R:g}, one volume per launch",
                    "grid": [side, side], "radius": R, "storage": "fp32", "op": "dilation",
                    "pass1_ms": round(p1_ms, 4), "pass2_ms": round(p2_ms, 4),
-                   "plain_fp64": plain, "pass2_sweep_tiles": fallback, "batched": batched, "served": served,
+                   "plain_fp64": plain, "pass2_sweep_tiles": fallback, "autotuned": tuned, "batched": batched,
+                   "served": served,
                    "s_in": round(s_in, 4), "s_out": round(s_out, 4), "m_oracle": m},
         "roofline": roofline,
         "cpu_baseline": cpu,

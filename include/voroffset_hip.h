@@ -126,6 +126,18 @@ int vo_mesh_shape(const vo_mesh_result* r, int64_t* num_vertices, int64_t* num_q
 int vo_mesh_copy(const vo_mesh_result* r, double* vertices, int64_t* quads);
 void vo_mesh_free(vo_mesh_result* r);
 
+/* Autotuner (no reference counterpart): for each radius (at most 127), times
+ * the z-sweep event capacity per slab (512, 768, 1024, 1536) and the gather
+ * list capacity (8, 16, 24, 32) on `sample` (a device volume of the given
+ * precision), `reps` runs each, and keeps the fastest per radius bucket
+ * ([0,8), [8,16), [16,32), [32,64), [64,128)) in `ctx`; results do not
+ * depend on the choice.  The table can be saved, loaded and read back. */
+int vo_autotune(vo_context* ctx, int precision, const vo_volume* sample, int num_radii, const double* radii,
+                int reps);
+int vo_tuning_save(const vo_context* ctx, const char* path);
+int vo_tuning_load(vo_context* ctx, const char* path);
+int vo_tuning_get(const vo_context* ctx, int bucket, int* events, int* ocap);
+
 /* Mesh -> dexels: replaces the ray casting of vor3d::create_dexels
  * (src/vor3d/Dexelize.cpp:166-225; predicates :56-162).  `vertices` is
  * num_vertices x 3 doubles (x, y, z), `triangles` num_triangles x 3 vertex

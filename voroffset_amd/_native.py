@@ -22,7 +22,8 @@ PRECISIONS = {"fp32": 0, "fp64": 1}
 EXPORTS = ("vo_create", "vo_destroy", "vo_last_error", "vo_morph", "vo_result_shape", "vo_result_copy",
            "vo_result_free", "vo_morph_device", "vo_last_stats", "vo_rasterize", "vo_morph_batch_device",
            "vo_last_batch_alone", "vo_measure_device", "vo_xor_device",
-           "vo_dexel_mesh", "vo_mesh_shape", "vo_mesh_copy", "vo_mesh_free")
+           "vo_dexel_mesh", "vo_mesh_shape", "vo_mesh_copy", "vo_mesh_free", "vo_autotune", "vo_tuning_save",
+           "vo_tuning_load", "vo_tuning_get")
 
 
 class VoError(RuntimeError):
@@ -90,6 +91,14 @@ def lib():
         L.vo_mesh_copy.restype = i32
         L.vo_mesh_free.argtypes = [vp]
         L.vo_mesh_free.restype = None
+        L.vo_autotune.argtypes = [vp, i32, ctypes.POINTER(VoVolume), i32, ctypes.POINTER(dbl), i32]
+        L.vo_autotune.restype = i32
+        L.vo_tuning_save.argtypes = [vp, ctypes.c_char_p]
+        L.vo_tuning_save.restype = i32
+        L.vo_tuning_load.argtypes = [vp, ctypes.c_char_p]
+        L.vo_tuning_load.restype = i32
+        L.vo_tuning_get.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.vo_tuning_get.restype = i32
         L.vo_last_batch_alone.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         L.vo_last_batch_alone.restype = i32
         _lib = L
@@ -232,6 +241,35 @@ class Context:
         finally:
             lib().vo_mesh_free(res)
         return vert, quad
+
+    TUNE_BUCKETS = 5
+
+    def autotune(self, precision, nx, ny, off_ptr, z_ptr, radii, reps=2):
+        """Measures and keeps the fastest shapes per radius bucket (device sample)."""
+        v = VoVolume(nx, ny, off_ptr, z_ptr, 0.0, 0.0)
+        r = (ctypes.c_double * max(len(radii), 1))(*map(float, radii))
+        rc = lib().vo_autotune(self._h, PRECISIONS[precision], ctypes.byref(v), len(radii), r, int(reps))
+        if rc != OK:
+            self._raise(rc)
+        return self.tuning()
+
+    def tuning(self):
+        """[(events, list capacity)] per radius bucket."""
+        out = []
+        for b in range(self.TUNE_BUCKETS):
+            e, o = ctypes.c_int(), ctypes.c_int()
+            lib().vo_tuning_get(self._h, b, ctypes.byref(e), ctypes.byref(o))
+            out.append((e.value, o.value))
+        return out
+
+    def tuning_save(self, path):
+        if lib().vo_tuning_save(self._h, str(path).encode()) != OK:
+            raise VoError(ERR_ARGUMENT, f"cannot write {path}")
+
+    def tuning_load(self, path):
+        rc = lib().vo_tuning_load(self._h, str(path).encode())
+        if rc != OK:
+            self._raise(rc)
 
     def last_batch_alone(self):
         """Jobs of the last batch that were redone by the single-volume path."""

@@ -37,7 +37,15 @@ constexpr int kP2SCap = 128, kP2PCap = 256, kP2NCap = 64, kP2OCap = 256;
 // gather form: intervals per output column list.  Large radii fuse the
 // dilated pieces into few intervals, so they get the short list (and more
 // waves per CU); a tile that overflows is recomputed by the sweep form.
-inline int gather_ocap(int rint) { return rint >= 6 ? 16 : 24; }
+// Tuning table: per radius bucket, the z-sweep event capacity per slab and
+// the gather list capacity (defaults below; vo_autotune measures them).
+constexpr int kTuneBuckets = 5;
+inline int tune_bucket(int rint) { return rint < 8 ? 0 : rint < 16 ? 1 : rint < 32 ? 2 : rint < 64 ? 3 : 4; }
+constexpr int kEventChoices[] = {512, 768, 1024, 1536};
+struct Tuning {
+  int events[kTuneBuckets] = {768, 768, 768, 768, 768};
+  int ocap[kTuneBuckets] = {24, 16, 16, 16, 16};
+};
 // pass-1 z-sweep form: LDS events per z-slab (denser windows take more slabs)
 // and histogram bins; small enough for ~5 waves per SIMD
 constexpr int kZECap = 768, kZNB = 256, kZWMax = 64 + 2 * 128;
@@ -109,7 +117,8 @@ struct vo_context {
   int p1_caps = 96;         // pass-1 slots per column (grows on overflow; a multiple of vo::kSlotBlock)
   int p2_cap_per_col = 16;  // pass-2 arena endpoints per column (grows on overflow)
   bool force_sweep = false;  // VO_PASS2=sweep: always use the sweep form of pass 2
-  int zsweep_events = 768;   // VO_ZSWEEP_EVENTS: z-sweep LDS event capacity (512, 768, 1024, 1536)
+  int zsweep_events = 0;     // VO_ZSWEEP_EVENTS: force a z-sweep event capacity (512, 768, 1024, 1536)
+  Tuning tuning;             // per radius bucket (vo_autotune, vo_tuning_load)
   int pass1_form = 0;        // VO_PASS1: 0 = z-sweep (default), 1 = "paint", 2 = "cull"
   int last_fallback_tiles = 0;
   std::vector<void*> batch_extra;  // outputs of batched jobs redone alone (freed by the next batch)
@@ -227,6 +236,53 @@ void launch_zsweep(vo_context* ctx, int rint, int64_t grid, const vo::Pass1ZArgs
     hipLaunchKernelGGL((vo::pass1_zsweep_kernel<T, ECAP, NB, kZWMax, 4>), g, b, 0, ctx->stream, az);
 }
 
+template <class T>
+void launch_zsweep_events(vo_context* ctx, int events, int rint, int64_t grid, const vo::Pass1ZArgs<T>& az) {
+  switch (events) {
+    case 512: launch_zsweep<T, 512, 128>(ctx, rint, grid, az); break;
+    case 1024: launch_zsweep<T, 1024, 256>(ctx, rint, grid, az); break;
+    case 1536: launch_zsweep<T, 1536, 512>(ctx, rint, grid, az); break;
+    default: launch_zsweep<T, 768, 256>(ctx, rint, grid, az); break;
+  }
+}
+
+// Batched z-sweep with the same event capacities.
+template <class T, int ECAP, int NB>
+void launch_zsweep_batch(vo_context* ctx, int max_rint, int64_t grid, const vo::Pass1ZArgs<T>* jobs,
+                         const int64_t* tile_base, int njobs) {
+  const dim3 g(static_cast<unsigned>(grid)), b(static_cast<unsigned>(vo::kWave));
+  if (max_rint <= 31)
+    hipLaunchKernelGGL((vo::pass1_zsweep_batch_kernel<T, ECAP, NB, kZWMax, 1>), g, b, 0, ctx->stream, jobs, tile_base,
+                       njobs);
+  else if (max_rint <= 63)
+    hipLaunchKernelGGL((vo::pass1_zsweep_batch_kernel<T, ECAP, NB, kZWMax, 2>), g, b, 0, ctx->stream, jobs, tile_base,
+                       njobs);
+  else
+    hipLaunchKernelGGL((vo::pass1_zsweep_batch_kernel<T, ECAP, NB, kZWMax, 4>), g, b, 0, ctx->stream, jobs, tile_base,
+                       njobs);
+}
+
+template <class T>
+void launch_zsweep_batch_events(vo_context* ctx, int events, int max_rint, int64_t grid,
+                                const vo::Pass1ZArgs<T>* jobs, const int64_t* tile_base, int njobs) {
+  switch (events) {
+    case 512: launch_zsweep_batch<T, 512, 128>(ctx, max_rint, grid, jobs, tile_base, njobs); break;
+    case 1024: launch_zsweep_batch<T, 1024, 256>(ctx, max_rint, grid, jobs, tile_base, njobs); break;
+    case 1536: launch_zsweep_batch<T, 1536, 512>(ctx, max_rint, grid, jobs, tile_base, njobs); break;
+    default: launch_zsweep_batch<T, 768, 256>(ctx, max_rint, grid, jobs, tile_base, njobs); break;
+  }
+}
+
+int events_for(const vo_context* ctx, int rint) {
+  return ctx->zsweep_events ? ctx->zsweep_events : ctx->tuning.events[tune_bucket(rint)];
+}
+
+// Small radii fuse little, so their lists keep at least 24 intervals.
+int ocap_for(const vo_context* ctx, int rint) {
+  const int o = ctx->tuning.ocap[tune_bucket(rint)];
+  return rint < 6 ? std::max(24, o) : o;
+}
+
 // One dilation: `in` -> CSR in (out_off_slot, out_z_slot).
 template <class T>
 DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int out_z_slot, double* t1,
@@ -268,13 +324,17 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
     // (list mode) over the gather tiles that overflowed, one column a block
     const bool gather = rint <= 128 && !ctx->force_sweep;
     const int64_t ntiles = int64_t(nx) * ((ny + vo::kGatherTile - 1) / vo::kGatherTile);
-    const int gocap = gather_ocap(rint);
-    const int64_t garena = gather ? int64_t(vo::kGatherTile) * 2 * gocap : 0;  // endpoints per gather tile
+    int gocap = ocap_for(ctx, rint);
     const int chunk2_eff = gather ? 1 : chunk2;
     const int64_t arena2 = int64_t(chunk2_eff) * ctx->p2_cap_per_col;  // endpoints per sweep block
     const int64_t sweep_blocks = gather ? ntiles * vo::kGatherTile : int64_t(ny) * ((nx + chunk2 - 1) / chunk2);
-    const int64_t o_total = ntiles * garena + sweep_blocks * arena2;
-    T* o_z = static_cast<T*>(ws(ctx, kO_Z, o_total * sizeof(T)));
+    int64_t garena = 0;  // endpoints per gather tile
+    T* o_z = nullptr;
+    auto size_arena = [&] {
+      garena = gather ? int64_t(vo::kGatherTile) * 2 * gocap : 0;
+      o_z = static_cast<T*>(ws(ctx, kO_Z, size_t(ntiles * garena + sweep_blocks * arena2) * sizeof(T)));
+    };
+    size_arena();
     int* tiles = static_cast<int*>(ws(ctx, kTILES, (ntiles + 1) * sizeof(int)));
     check(ctx, hipMemsetAsync(status, 0, sizeof(int), ctx->stream), "memset status");
     check(ctx, hipMemsetAsync(tiles, 0, sizeof(int), ctx->stream), "memset tiles");
@@ -289,12 +349,7 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
       vo::Pass1ZArgs<T> az{nx, ny, rint, in.off, static_cast<const T*>(in.z), m_lo, m_hi, m_dy, caps, m_cnt,
                            status};
       const int64_t gridz = int64_t(nx) * ((ny + vo::kZTile - 1) / vo::kZTile);
-      switch (ctx->zsweep_events) {
-        case 512: launch_zsweep<T, 512, 128>(ctx, rint, gridz, az); break;
-        case 1024: launch_zsweep<T, 1024, 256>(ctx, rint, gridz, az); break;
-        case 1536: launch_zsweep<T, 1536, 512>(ctx, rint, gridz, az); break;
-        default: launch_zsweep<T, kZECap, kZNB>(ctx, rint, gridz, az); break;
-      }
+      launch_zsweep_events<T>(ctx, events_for(ctx, rint), rint, gridz, az);
     } else if (ctx->pass1_form == 2) {
       hipLaunchKernelGGL((vo::pass1_kernel<T, kP1Cap, kP1NCap, Shapes<T>::p1_arena, Shapes<T>::p1_cmax>),
                          dim3(unsigned(grid1)), dim3(vo::kWave), 0, ctx->stream, a1);
@@ -306,18 +361,27 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
     check(ctx, hipEventRecord(ctx->ev[1], ctx->stream), "event");
     int nfall = 0;
     if (gather) {
-      double* lists = static_cast<double*>(ws(ctx, kG_LISTS, size_t(ntiles) * 2 * gocap * vo::kWave * sizeof(double)));
       const double* ht = upload_htab(ctx, kG_HTAB, {R}, htab_host);
-      vo::Pass2GArgs<T> ag{nx, ny, rint, R2, m_lo, m_hi, m_dy, caps, m_cnt, o_z, garena, o_start, o_len,
-                           tiles, status, gocap, ht, 0, lists};
-      if (rint <= vo::kGatherLdsTableMax)
-        hipLaunchKernelGGL((vo::pass2_gather_kernel<T, true>), dim3(unsigned(ntiles)), dim3(vo::kWave),
-                           size_t(rint + 1) * (rint + 1) * sizeof(double), ctx->stream, ag);
-      else
-        hipLaunchKernelGGL((vo::pass2_gather_kernel<T, false>), dim3(unsigned(ntiles)), dim3(vo::kWave), 0,
-                           ctx->stream, ag);
-      check(ctx, hipGetLastError(), "pass2 gather launch");
-      nfall = int(read_word32(ctx, tiles));
+      for (;;) {
+        double* lists =
+            static_cast<double*>(ws(ctx, kG_LISTS, size_t(ntiles) * 2 * gocap * vo::kWave * sizeof(double)));
+        vo::Pass2GArgs<T> ag{nx, ny, rint, R2, m_lo, m_hi, m_dy, caps, m_cnt, o_z, garena, o_start, o_len,
+                             tiles, status, gocap, ht, 0, lists};
+        if (rint <= vo::kGatherLdsTableMax)
+          hipLaunchKernelGGL((vo::pass2_gather_kernel<T, true>), dim3(unsigned(ntiles)), dim3(vo::kWave),
+                             size_t(rint + 1) * (rint + 1) * sizeof(double), ctx->stream, ag);
+        else
+          hipLaunchKernelGGL((vo::pass2_gather_kernel<T, false>), dim3(unsigned(ntiles)), dim3(vo::kWave), 0,
+                             ctx->stream, ag);
+        check(ctx, hipGetLastError(), "pass2 gather launch");
+        nfall = int(read_word32(ctx, tiles));
+        if (nfall == 0 || gocap >= 64) break;
+        // lists outgrew their capacity: the whole gather again with twice the
+        // room is cheaper and more robust than the sweep form for many tiles
+        gocap *= 2;
+        size_arena();
+        check(ctx, hipMemsetAsync(tiles, 0, sizeof(int), ctx->stream), "memset tiles");
+      }
     }
     // the sweep form: the whole volume, or the tiles the gather form gave back
     const int64_t grid2 = gather ? int64_t(nfall) * vo::kGatherTile : sweep_blocks;
@@ -500,7 +564,7 @@ void dilate_batch(vo_context* ctx, std::vector<DevVol>& vol, const std::vector<d
       col_base[k + 1] = col_base[k] + int64_t(v.nx) * v.ny;
       tile_base[k + 1] = tile_base[k] + int64_t(v.nx) * ((v.ny + vo::kGatherTile - 1) / vo::kGatherTile);
       max_rint = std::max(max_rint, rint);
-      ocap = std::max(ocap, gather_ocap(rint));
+      ocap = std::max(ocap, ocap_for(ctx, rint));
     }
     std::vector<double> radii(K);
     std::vector<int64_t> htab_base(K, 0);
@@ -546,15 +610,7 @@ void dilate_batch(vo_context* ctx, std::vector<DevVol>& vol, const std::vector<d
     check(ctx, hipMemsetAsync(o_len, 0, size_t(C + 1) * sizeof(int64_t), ctx->stream), "memset lengths");
     check(ctx, hipEventRecord(ctx->ev[0], ctx->stream), "event");
     const dim3 grid(static_cast<unsigned>(NT)), block(static_cast<unsigned>(vo::kWave));
-    if (max_rint <= 31)
-      hipLaunchKernelGGL((vo::pass1_zsweep_batch_kernel<T, kZECap, kZNB, kZWMax, 1>), grid, block, 0, ctx->stream, d1,
-                         dt, K);
-    else if (max_rint <= 63)
-      hipLaunchKernelGGL((vo::pass1_zsweep_batch_kernel<T, kZECap, kZNB, kZWMax, 2>), grid, block, 0, ctx->stream, d1,
-                         dt, K);
-    else
-      hipLaunchKernelGGL((vo::pass1_zsweep_batch_kernel<T, kZECap, kZNB, kZWMax, 4>), grid, block, 0, ctx->stream, d1,
-                         dt, K);
+    launch_zsweep_batch_events<T>(ctx, events_for(ctx, max_rint), max_rint, NT, d1, dt, K);
     check(ctx, hipGetLastError(), "batched pass1 launch");
     check(ctx, hipEventRecord(ctx->ev[1], ctx->stream), "event");
     if (max_rint <= vo::kGatherLdsTableMax)
@@ -882,6 +938,67 @@ void dexel_mesh(vo_context* ctx, const vo_volume* v, double ox, double oy, doubl
   }
   check(ctx, hipStreamSynchronize(ctx->stream), "sync");
   *out = res.release();
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: per radius, the z-sweep event capacity (pass-1 time) and the
+// gather list capacity (pass-2 time, fallbacks included) measured on a sample
+// volume; the fastest of each is kept for the radius' bucket.
+
+template <class T>
+void autotune(vo_context* ctx, const vo_volume* sample, int n, const double* radii, int reps) {
+  validate_shape(ctx, sample);
+  if (n < 0 || (n > 0 && !radii) || reps < 1 || reps > 100) fail(ctx, VO_ERR_ARGUMENT, "vo_autotune: bad arguments");
+  DevVol in;
+  in.nx = sample->nx;
+  in.ny = sample->ny;
+  in.off = sample->offsets;
+  in.z = sample->z;
+  const int forced = ctx->zsweep_events;
+  for (int i = 0; i < n; ++i) {
+    const double R = radii[i];
+    if (!(R >= 0) || !std::isfinite(R) || R > 127) fail(ctx, VO_ERR_ARGUMENT, "vo_autotune: radius out of [0, 127]");
+    const int b = tune_bucket(int(std::floor(R)));
+    auto time_of = [&](int which) {
+      double best = INFINITY;
+      for (int r = 0; r < reps; ++r) {
+        double t1 = 0, t2 = 0;
+        dilate<T>(ctx, in, R, kV3_OFF, kV3_Z, &t1, &t2);
+        best = std::min(best, which == 1 ? t1 : t2);
+      }
+      return best;
+    };
+    // a candidate replaces the current choice only when it is more than 3 %
+    // faster, so timing noise on a small sample does not move the defaults
+    const int cur_e = ctx->tuning.events[b];
+    ctx->zsweep_events = cur_e;
+    int best_e = cur_e;
+    double tb = time_of(1);
+    for (int e : kEventChoices) {
+      if (e == cur_e) continue;
+      ctx->zsweep_events = e;
+      const double t = time_of(1);
+      if (t < 0.97 * tb) {
+        tb = t;
+        best_e = e;
+      }
+    }
+    ctx->zsweep_events = forced;
+    ctx->tuning.events[b] = best_e;
+    const int cur_o = ctx->tuning.ocap[b];
+    int best_o = cur_o;
+    tb = time_of(2);
+    for (int o : {8, 16, 24, 32}) {
+      if (o == cur_o) continue;
+      ctx->tuning.ocap[b] = o;
+      const double t = time_of(2);
+      if (t < 0.97 * tb) {
+        tb = t;
+        best_o = o;
+      }
+    }
+    ctx->tuning.ocap[b] = best_o;
+  }
 }
 
 int guarded(vo_context* ctx, const std::function<void()>& body) {
@@ -1221,6 +1338,60 @@ int vo_mesh_copy(const vo_mesh_result* r, double* vertices, int64_t* quads) {
 }
 
 void vo_mesh_free(vo_mesh_result* r) { delete r; }
+
+int vo_autotune(vo_context* ctx, int precision, const vo_volume* sample, int num_radii, const double* radii, int reps) {
+  return guarded(ctx, [&] {
+    if (precision == VO_FP32)
+      autotune<float>(ctx, sample, num_radii, radii, reps);
+    else if (precision == VO_FP64)
+      autotune<double>(ctx, sample, num_radii, radii, reps);
+    else
+      fail(ctx, VO_ERR_ARGUMENT, "unknown precision");
+  });
+}
+
+int vo_tuning_get(const vo_context* ctx, int bucket, int* events, int* ocap) {
+  if (!ctx || bucket < 0 || bucket >= kTuneBuckets) return VO_ERR_ARGUMENT;
+  if (events) *events = ctx->tuning.events[bucket];
+  if (ocap) *ocap = ctx->tuning.ocap[bucket];
+  return VO_OK;
+}
+
+int vo_tuning_save(const vo_context* ctx, const char* path) {
+  if (!ctx || !path) return VO_ERR_ARGUMENT;
+  FILE* f = std::fopen(path, "w");
+  if (!f) return VO_ERR_ARGUMENT;
+  std::fprintf(f, "voroffset_amd tuning 1\n");
+  for (int b = 0; b < kTuneBuckets; ++b) std::fprintf(f, "%d %d %d\n", b, ctx->tuning.events[b], ctx->tuning.ocap[b]);
+  return std::fclose(f) == 0 ? VO_OK : VO_ERR_ARGUMENT;
+}
+
+int vo_tuning_load(vo_context* ctx, const char* path) {
+  if (!ctx || !path) return VO_ERR_ARGUMENT;
+  FILE* f = std::fopen(path, "r");
+  if (!f) {
+    ctx->error = std::string("cannot open ") + path;
+    return VO_ERR_ARGUMENT;
+  }
+  Tuning t = ctx->tuning;
+  int version = 0, ok = std::fscanf(f, "voroffset_amd tuning %d", &version) == 1 && version == 1;
+  for (int i = 0; ok && i < kTuneBuckets; ++i) {
+    int b, e, o;
+    ok = std::fscanf(f, "%d %d %d", &b, &e, &o) == 3 && b >= 0 && b < kTuneBuckets &&
+         std::count(std::begin(kEventChoices), std::end(kEventChoices), e) == 1 && o >= 8 && o <= 64;
+    if (ok) {
+      t.events[b] = e;
+      t.ocap[b] = o;
+    }
+  }
+  std::fclose(f);
+  if (!ok) {
+    ctx->error = std::string("malformed tuning file ") + path;
+    return VO_ERR_ARGUMENT;
+  }
+  ctx->tuning = t;
+  return VO_OK;
+}
 
 int vo_last_batch_alone(const vo_context* ctx, int* jobs) {
   if (!ctx || !jobs) return VO_ERR_ARGUMENT;
