@@ -67,6 +67,10 @@ __device__ __forceinline__ int64_t slot_index(int ny, int caps, int x, int y, in
   return ((int64_t(x) * (caps / kSlotBlock) + j / kSlotBlock) * ny + y) * kSlotBlock + (j % kSlotBlock);
 }
 __device__ __forceinline__ int64_t count_index(int ny, int x, int y) { return int64_t(x) * ny + y; }
+// Summary of the block holding triple j of column (x, y) (same y-minor order).
+__device__ __forceinline__ int64_t summary_index(int ny, int caps, int x, int y, int j) {
+  return (int64_t(x) * (caps / kSlotBlock) + j / kSlotBlock) * ny + y;
+}
 
 // Batched launches: job j owns the global tiles (or columns)
 // [base[j], base[j + 1]); returns the j holding t (base[njobs] = total).
@@ -928,6 +932,8 @@ struct Pass2GArgs {
   const double* htab;   // H by (|d|, dy): htab[|d| * (rint + 1) + dy], -1 out of reach
   int64_t o_base;       // first endpoint of this volume's tiles in o_z (batched launches)
   double* lists;        // interval lists, 2 * ocap * 64 doubles per tile (lane-minor)
+  const T* s_lo;        // block summaries from pass 1 (null: none, no block is skipped)
+  const T* s_hi;
 };
 
 
@@ -1063,64 +1069,80 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
       if (slow) unite_slow(s, e);
     }
   };
-  // Blocks of B pieces (j) outermost, source columns nearest first within a
-  // block (d = 0, -1, +1, -2, ...): a block touches a few cache lines per
-  // source column, which the neighbouring tiles read at about the same time.
-  // Within a source column the pieces are z-sorted, and touching dilated
-  // pieces fuse into a run in registers before they reach the list.  Loads
-  // are unconditional from clamped in-bounds slots (values past a column's
-  // count are read and discarded), so the inner loop needs no branches.
   constexpr int B = kSlotBlock;
-  for (int jb = 0; jb < jmax; jb += B) {
+  // The lane's block of B contiguous triples of column c from jb on (jb is a
+  // multiple of B and below caps, so the whole block is inside the column's
+  // slots; values past the column's count are read and discarded): dilated
+  // and fused into the open run, which is united when a gap opens.  Loads are
+  // unconditional and the loop is written with selects, so it has no branches.
+  auto process_block = [&](int c, int jb, int m, const double* hrow, bool& run, double& rs, double& re) {
+    const int64_t o = slot_index(g.ny, g.caps, c, yc, jb);
+    T va[B], vb[B];
+    int vdy[B];
+    load_block(g.m_lo + o, va);
+    load_block(g.m_hi + o, vb);
+    {
+      const uint4 q = *reinterpret_cast<const uint4*>(g.m_dy + o);
+      const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+      for (int u = 0; u < B; ++u) vdy[u] = int((w[u >> 1] >> (16 * (u & 1))) & 0xffffu);
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int dy = min(vdy[u], rint);
+      const double h = hrow[dy];
+      const bool valid = ok && jb + u < m && vdy[u] <= rint && h >= 0;
+      const double s = double(va[u]) - h, e = double(vb[u]) + h;
+      // triples are z-sorted, so e >= rs whenever s <= re: the run stays connected
+      const bool ext = valid && run && s <= re;
+      unite(valid && run && !ext, rs, re);
+      rs = ext ? fmin(rs, s) : (valid ? s : rs);
+      re = ext ? fmax(re, e) : (valid ? e : re);
+      run = run || valid;
+    }
+  };
+  if (g.s_lo) {
+    // Source columns nearest first (d = 0, -1, +1, -2, ...), their blocks in
+    // z order.  The near columns build most of the union; a farther block
+    // whose summary range, widened by the largest dilation at that distance
+    // H(d, 0), lies inside every lane's cached interval changes nothing and
+    // is neither loaded nor processed.
     for (int step = 0; step <= 2 * rint; ++step) {
       const int ad = (step + 1) >> 1;
       const int d = (step & 1) ? -ad : ad;
       const int c = x + d;
       if (c < 0 || c >= g.nx) continue;  // uniform
       const int m = count_of(c);
-      if (ballot(m > jb && ok) == 0) continue;
       const double* hrow = htab + ad * (rint + 1);
-      // the lane's block: B contiguous triples (jb is a multiple of B and
-      // below caps, so the whole block is inside the column's slots)
-      const int64_t o = slot_index(g.ny, g.caps, c, yc, jb);
-      T va[B], vb[B];
-      int vdy[B];
-      load_block(g.m_lo + o, va);
-      load_block(g.m_hi + o, vb);
-      {
-        const uint4 q = *reinterpret_cast<const uint4*>(g.m_dy + o);
-        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-        for (int u = 0; u < B; ++u) vdy[u] = int((w[u >> 1] >> (16 * (u & 1))) & 0xffffu);
-      }
-      // the block's triples lie in [a_first, b_last] (z-sorted); dilated by at
-      // most H(d, 0) they stay inside that range widened by it: when the
-      // cached interval holds it for every lane, the block changes nothing
-      {
-        const int nv = min(m - jb, B);  // valid triples of this lane's block (<= 0: none)
-        double blast = double(vb[0]);
-#pragma unroll
-        for (int u = 1; u < B; ++u) blast = u < nv ? double(vb[u]) : blast;
-        const double hmax = hrow[0];
-        const bool inside = nv <= 0 || !ok || (double(va[0]) - hmax >= clo && blast + hmax <= chi);
-        if (ballot(!inside) == 0) continue;
-      }
+      const double hmax = hrow[0];
       bool run = false;
       double rs = 0, re = 0;
-#pragma unroll
-      for (int u = 0; u < B; ++u) {
-        const int dy = min(vdy[u], rint);
-        const double h = hrow[dy];
-        const bool valid = ok && jb + u < m && vdy[u] <= rint && h >= 0;
-        const double s = double(va[u]) - h, e = double(vb[u]) + h;
-        // pieces are z-sorted, so e >= rs whenever s <= re: the run stays connected
-        const bool ext = valid && run && s <= re;
-        unite(valid && run && !ext, rs, re);
-        rs = ext ? fmin(rs, s) : (valid ? s : rs);
-        re = ext ? fmax(re, e) : (valid ? e : re);
-        run = run || valid;
+      for (int jb = 0; ballot(m > jb && ok) != 0; jb += B) {
+        const int64_t si = summary_index(g.ny, g.caps, c, yc, jb);
+        const double blo = double(g.s_lo[si]), bhi = double(g.s_hi[si]);
+        const bool inside = jb >= m || !ok || (blo - hmax >= clo && bhi + hmax <= chi);
+        if (ballot(!inside) == 0) continue;
+        process_block(c, jb, m, hrow, run, rs, re);
       }
       unite(run && ok, rs, re);
+    }
+  } else {
+    // Without summaries: blocks outermost, source columns nearest first
+    // within a block, so a block's cache lines are read by all its steps
+    // at about the same time.
+    for (int jb = 0; jb < jmax; jb += B) {
+      for (int step = 0; step <= 2 * rint; ++step) {
+        const int ad = (step + 1) >> 1;
+        const int d = (step & 1) ? -ad : ad;
+        const int c = x + d;
+        if (c < 0 || c >= g.nx) continue;  // uniform
+        const int m = count_of(c);
+        if (ballot(m > jb && ok) == 0) continue;
+        bool run = false;
+        double rs = 0, re = 0;
+        process_block(c, jb, m, htab + ad * (rint + 1), run, rs, re);
+        unite(run && ok, rs, re);
+      }
     }
   }
   if (dirty && ok) {
@@ -1460,6 +1482,10 @@ struct Pass1ZArgs {
   int caps;
   int32_t* m_cnt;
   int* status;
+  // block summaries at summary_index: lo of the block's first triple, hi of
+  // its last one
+  T* s_lo;
+  T* s_hi;
 };
 
 constexpr int kZTile = kWave;    // output columns per wavefront
@@ -1508,6 +1534,8 @@ struct ZState {
   bool point_open = false, ok = true;
   int64_t slot = 0;    // slot of triple `count` of this lane's column
   int block_jump = 0;  // slot step from the last triple of a block to the next block
+  int64_t sidx = 0;    // summary of the block of triple `count`
+  int ny = 0;
 
   __device__ __forceinline__ explicit ZState(int r) : R(r) {
 #pragma unroll
@@ -1602,11 +1630,14 @@ struct ZState {
       g.m_lo[slot] = z0;
       g.m_hi[slot] = z;
       g.m_dy[slot] = uint16_t(dcur);
+      if (count % kSlotBlock == 0) g.s_lo[sidx] = z0;
+      g.s_hi[sidx] = z;
     }
     ok = ok && !(put && !fits);
     count += put ? 1 : 0;
-    const int step = put ? ((count % kSlotBlock == 0) ? block_jump : 1) : 0;
-    slot += step;
+    const bool next_block = put && count % kSlotBlock == 0;
+    slot += put ? (next_block ? block_jump : 1) : 0;
+    sidx += next_block ? ny : 0;
     const bool open = changed && dist <= R;
     z0 = open ? z : z0;
     point_open = open ? is_point : point_open;
@@ -1710,6 +1741,8 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
   ZState<NW, T> st(R);
   st.slot = slot_index(g.ny, g.caps, x, y0 + L, 0);
   st.block_jump = g.ny * kSlotBlock - (kSlotBlock - 1);
+  st.sidx = summary_index(g.ny, g.caps, x, y0 + L, 0);
+  st.ny = g.ny;
   int b_lo = 0;
   for (int sl = 0; sl < nslab; ++sl) {
     const int b_hi = sm.slab_end[sl];
