@@ -347,7 +347,8 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
     vo::Pass1Args<T> a1{nx, ny, rint, chunk1, nch1, in.off, static_cast<const T*>(in.z), m_lo, m_hi, m_dy,
                         caps, m_cnt, status};
     check(ctx, hipEventRecord(ctx->ev[0], ctx->stream), "event");
-    const bool zsweep = ctx->pass1_form == 0 && rint <= 127 && !zsweep_failed;
+    const bool zsweep = ctx->pass1_form == 0 && rint <= 127 && !zsweep_failed &&
+                        vo::zsweep_offsets_fit(caps, ny, int(sizeof(T)));
     if (zsweep) {
       vo::Pass1ZArgs<T> az{nx, ny, rint, in.off, static_cast<const T*>(in.z), m_lo, m_hi, m_dy, caps, m_cnt,
                            status, s_lo, s_hi};
@@ -552,7 +553,8 @@ void dilate_batch(vo_context* ctx, std::vector<DevVol>& vol, const std::vector<d
   for (int j : idx) {
     if (!(R[j] >= 0) || !std::isfinite(R[j]) || R[j] > 60000) fail(ctx, VO_ERR_ARGUMENT, "radius must be finite and >= 0");
     const int rint = int(std::floor(R[j]));
-    if (rint > 127 || int64_t(vol[j].nx) * vol[j].ny == 0 || ctx->force_sweep || ctx->pass1_form != 0)
+    if (rint > 127 || int64_t(vol[j].nx) * vol[j].ny == 0 || ctx->force_sweep || ctx->pass1_form != 0 ||
+        !vo::zsweep_offsets_fit(ctx->p1_caps, vol[j].ny, int(sizeof(T))))
       alone.push_back(j);
     else
       jobs.push_back(j);

@@ -1489,7 +1489,15 @@ struct Pass1ZArgs {
 };
 
 constexpr int kZTile = kWave;    // output columns per wavefront
-constexpr int kGroupEnd = 1 << 30;  // event flag: last of its (z, kind) group
+constexpr int kWalkMaxR = 40;    // largest R whose lanes walk their own events
+
+// The z-sweep stores a wavefront's triples and summaries at 32-bit byte
+// offsets from the wavefront's first slot and summary (a scalar base plus a
+// vector offset); true when every offset fits.  The largest slot offset is
+// below caps * ny + 512.
+__host__ __device__ constexpr bool zsweep_offsets_fit(int caps, int ny, int tsize) {
+  return (int64_t(caps) * ny + 1024) * tsize < (int64_t(1) << 32);
+}
 constexpr int kMaxSlabs = 64;    // z-slabs per window
 
 template <class T, int ECAP, int NB, int WMAX>
@@ -1525,6 +1533,12 @@ __device__ __forceinline__ void column_segments(const T* z, int64_t s, int64_t n
 // the open piece.
 // The sweep only compares endpoint values it read, so the open piece's start
 // is kept in the storage type T.
+// base[i] as a scalar base plus a 32-bit vector byte offset
+template <class V>
+__device__ __forceinline__ void store_at(V* base, uint32_t i, V v) {
+  *reinterpret_cast<V*>(reinterpret_cast<char*>(base) + i * uint32_t(sizeof(V))) = v;
+}
+
 template <int NW, class T>
 struct ZState {
   uint64_t w[NW];
@@ -1532,28 +1546,30 @@ struct ZState {
   int dist = 1 << 20, dcur = 1 << 20, count = 0;
   T z0 = T(0);
   bool point_open = false, ok = true;
-  int64_t slot = 0;    // slot of triple `count` of this lane's column
-  int block_jump = 0;  // slot step from the last triple of a block to the next block
-  int64_t sidx = 0;    // summary of the block of triple `count`
-  int ny = 0;
+  // the wavefront's first slot and summary in each array (uniform), and this
+  // lane's slot and summary offsets from them for triple 0
+  T* lo_w = nullptr;
+  T* hi_w = nullptr;
+  uint16_t* dy_w = nullptr;
+  T* slo_w = nullptr;
+  T* shi_w = nullptr;
+  uint32_t slot0 = 0, sidx0 = 0;
+  uint32_t ny = 0;
+  // the lane's pending group of points of equal z (their bits stay set until
+  // the lane's next event that is not a point at the same z)
+  bool pend = false;
+  T pz = T(0);
+  uint64_t gpts[NW];
 
   __device__ __forceinline__ explicit ZState(int r) : R(r) {
 #pragma unroll
-    for (int t = 0; t < NW; ++t) w[t] = 0;
+    for (int t = 0; t < NW; ++t) w[t] = gpts[t] = 0;
   }
   __device__ __forceinline__ uint64_t word(int i) const {
     uint64_t v = 0;
 #pragma unroll
     for (int t = 0; t < NW; ++t) v = t == i ? w[t] : v;
     return v;
-  }
-  __device__ __forceinline__ void flip(int k, bool on) {
-    const uint64_t b = uint64_t(1) << (k & 63);
-    const int i = k >> 6;
-#pragma unroll
-    for (int t = 0; t < NW; ++t) {
-      if (t == i) w[t] = on ? (w[t] | b) : (w[t] & ~b);
-    }
   }
   // distance from bit R to the nearest set bit, or R + 1 when none is set
   __device__ __forceinline__ int nearest() const {
@@ -1586,62 +1602,112 @@ struct ZState {
     }
     return best;
   }
-  // The updates below are written with selects: every lane runs them for
-  // every event, and only the store of a finished piece is predicated.
-  __device__ __forceinline__ void set(int k) {
-    const bool in = unsigned(k) <= unsigned(2 * R);
-    const int kk = in ? k : R;
-    const uint64_t b = in ? (uint64_t(1) << (kk & 63)) : 0;
-    if constexpr (NW == 1) {
-      w[0] |= b;  // 2R + 1 <= 64: every in-range bit is in word 0
-    } else {
-#pragma unroll
-      for (int t = 0; t < NW; ++t) w[t] |= (t == (kk >> 6)) ? b : 0;
-    }
-    const int dd = kk > R ? kk - R : R - kk;
-    dist = (in && dd < dist) ? dd : dist;
-  }
-  __device__ __forceinline__ void clear(int k) {
-    const bool in = unsigned(k) <= unsigned(2 * R);
-    const int kk = in ? k : R;
-    const uint64_t b = in ? (uint64_t(1) << (kk & 63)) : 0;
-    if constexpr (NW == 1) {
-      w[0] &= ~b;
-    } else {
-#pragma unroll
-      for (int t = 0; t < NW; ++t) w[t] &= (t == (kk >> 6)) ? ~b : ~uint64_t(0);
-    }
-    const int dd = kk > R ? kk - R : R - kk;
-    const bool again = in && dd == dist;
-    if constexpr (NW == 1) {
-      const int nd = nearest();  // a few instructions: cheaper than a branch
-      dist = again ? nd : dist;
-    } else {
-      if (again) dist = nearest();
-    }
-  }
+  // The updates below are written with selects, predicated by `on`: lanes
+  // walk their own events in lock step, and only the store of a finished
+  // piece is a predicated branch.
+  //
+  // Starts and ends are applied one at a time: within a group of equal
+  // (z, kind) the distance moves one way only, so the pieces a group would
+  // close at once are closed at its first change and the pieces in between
+  // have zero length and are not kept.  Points are grouped: all points of
+  // one z are in place together for one zero-length piece.
+
   // close the open piece and open the next one when the distance changed;
   // zero-length pieces are kept only when a point opened them
-  __device__ __forceinline__ void emit(T z, bool is_point, bool active, const Pass1ZArgs<T>& g) {
-    const bool changed = dist != dcur;
+  __device__ __forceinline__ void emit(T z, bool is_point, bool on, bool active, const Pass1ZArgs<T>& g) {
+    const bool changed = on && dist != dcur;
     const bool put = changed && dcur <= R && (z0 < z || point_open) && active;
-    const bool fits = count < g.caps;
-    if (put && fits) {
-      g.m_lo[slot] = z0;
-      g.m_hi[slot] = z;
-      g.m_dy[slot] = uint16_t(dcur);
-      if (count % kSlotBlock == 0) g.s_lo[sidx] = z0;
-      g.s_hi[sidx] = z;
+    if (put && count < g.caps) {
+      const uint32_t j = uint32_t(count);
+      const uint32_t blk = j / kSlotBlock;
+      const uint32_t s = slot0 + blk * (ny * kSlotBlock) + j % kSlotBlock;
+      const uint32_t q = sidx0 + blk * ny;
+      store_at(lo_w, s, z0);
+      store_at(hi_w, s, z);
+      store_at(dy_w, s, uint16_t(dcur));
+      if (j % kSlotBlock == 0) store_at(slo_w, q, z0);
+      store_at(shi_w, q, z);
     }
-    ok = ok && !(put && !fits);
+    ok = ok && !(put && count >= g.caps);
     count += put ? 1 : 0;
-    const bool next_block = put && count % kSlotBlock == 0;
-    slot += put ? (next_block ? block_jump : 1) : 0;
-    sidx += next_block ? ny : 0;
     const bool open = changed && dist <= R;
     z0 = open ? z : z0;
     point_open = open ? is_point : point_open;
-    dcur = dist;
+    dcur = on ? dist : dcur;
+  }
+  // a finished point group: pieces are closed at its z with the points in
+  // place, the points withdrawn and the piece after them opened at that z
+  __device__ __forceinline__ void close_points(bool on, bool active, const Pass1ZArgs<T>& g) {
+    emit(pz, true, on, active, g);
+#pragma unroll
+    for (int t = 0; t < NW; ++t) w[t] &= on ? ~gpts[t] : ~uint64_t(0);
+    const int nd = nearest();
+    dist = on ? nd : dist;
+    emit(pz, false, on, active, g);
+#pragma unroll
+    for (int t = 0; t < NW; ++t) gpts[t] = on ? 0 : gpts[t];
+    pend = pend && !on;
+  }
+  // one event of a window column within reach (bit k, 0 <= k <= 2R) where
+  // `on`; kind 0 start, 1 point, 2 end
+  __device__ __forceinline__ void apply(T z, int kind, int k, bool on, bool active, const Pass1ZArgs<T>& g) {
+    const bool point = kind == 1;
+    const bool flush = on && pend && !(point && z == pz);
+    if (ballot(flush) != 0) close_points(flush, active, g);
+    const int kk = on ? k : R;
+    const uint64_t b = on ? (uint64_t(1) << (kk & 63)) : 0;
+    const bool end = kind == 2;
+#pragma unroll
+    for (int t = 0; t < NW; ++t) {
+      const uint64_t bt = (NW == 1 || t == (kk >> 6)) ? b : 0;
+      w[t] = end ? (w[t] & ~bt) : (w[t] | bt);
+      gpts[t] |= point ? bt : 0;
+    }
+    const int dd = kk > R ? kk - R : R - kk;
+    dist = (on && !end && dd < dist) ? dd : dist;
+    const bool again = on && end && dd == dist;  // the cleared column was this lane's nearest
+    if (ballot(again) != 0) {
+      const int nd = nearest();
+      dist = again ? nd : dist;
+    }
+    pz = (on && point) ? z : pz;
+    pend = pend || (on && point);
+    emit(z, false, on && !point, active, g);
+  }
+  // the same for an event broadcast to every lane: `kind` is uniform, so the
+  // kinds take branches of their own
+  __device__ __forceinline__ void apply_all(T z, int kind, int k, bool on, bool active, const Pass1ZArgs<T>& g) {
+    const bool point = kind == 1;
+    const bool flush = on && pend && !(point && z == pz);
+    if (ballot(flush) != 0) close_points(flush, active, g);
+    const int kk = on ? k : R;
+    const uint64_t b = on ? (uint64_t(1) << (kk & 63)) : 0;
+    const int dd = kk > R ? kk - R : R - kk;
+    if (kind == 2) {
+#pragma unroll
+      for (int t = 0; t < NW; ++t) w[t] &= (NW == 1 || t == (kk >> 6)) ? ~b : ~uint64_t(0);
+      const bool again = on && dd == dist;
+      if constexpr (NW == 1) {
+        const int nd = nearest();  // a few instructions: cheaper than a branch
+        dist = again ? nd : dist;
+      } else if (ballot(again) != 0) {
+        const int nd = nearest();
+        dist = again ? nd : dist;
+      }
+      emit(z, false, on, active, g);
+    } else {
+#pragma unroll
+      for (int t = 0; t < NW; ++t) w[t] |= (NW == 1 || t == (kk >> 6)) ? b : 0;
+      dist = (on && dd < dist) ? dd : dist;
+      if (point) {
+#pragma unroll
+        for (int t = 0; t < NW; ++t) gpts[t] |= (NW == 1 || t == (kk >> 6)) ? b : 0;
+        pz = on ? z : pz;
+        pend = pend || on;
+      } else {
+        emit(z, false, on, active, g);
+      }
+    }
   }
 };
 
@@ -1739,10 +1805,16 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
     return;
   }
   ZState<NW, T> st(R);
-  st.slot = slot_index(g.ny, g.caps, x, y0 + L, 0);
-  st.block_jump = g.ny * kSlotBlock - (kSlotBlock - 1);
-  st.sidx = summary_index(g.ny, g.caps, x, y0 + L, 0);
-  st.ny = g.ny;
+  const int64_t wslot = slot_index(g.ny, g.caps, x, y0, 0);
+  const int64_t wsum = summary_index(g.ny, g.caps, x, y0, 0);
+  st.lo_w = g.m_lo + wslot;
+  st.hi_w = g.m_hi + wslot;
+  st.dy_w = g.m_dy + wslot;
+  st.slo_w = g.s_lo + wsum;
+  st.shi_w = g.s_hi + wsum;
+  st.slot0 = uint32_t(L * kSlotBlock);
+  st.sidx0 = uint32_t(L);
+  st.ny = uint32_t(g.ny);
   int b_lo = 0;
   for (int sl = 0; sl < nslab; ++sl) {
     const int b_hi = sm.slab_end[sl];
@@ -1813,41 +1885,48 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
       }
     }
     wave_sync();
-    // mark the last event of every group of equal (z, kind) (the slab's last
-    // event ends a group); the flag leaves the low bits alone, which is all a
-    // neighbouring lane reads here
-    for (int i = L; i < E; i += kWave) {
-      const bool end = i + 1 == E || sm.ez[i + 1] != sm.ez[i] || (sm.ep[i + 1] & 3) != (sm.ep[i] & 3);
-      if (end) sm.ep[i] |= kGroupEnd;
-    }
-    wave_sync();
-    // sweep in (z, kind) order; a group of equal (z, kind) is applied whole
-    // before the lanes compare distances.  Events are read 64 at a time into
-    // registers and broadcast with readlane.
-    int gstart = 0;
+    // Sweep in (z, kind) order.  Events are read 64 at a time into registers;
+    // a lane's events are those of the 2R + 1 window columns within its reach
+    // (k = column - L in [0, 2R]).  For R <= kWalkMaxR each lane marks its
+    // events of the chunk and, when no lane has more than 3/4 of the chunk
+    // marked, the lanes walk their own marked events in lock step, fetching
+    // each with a lane permute.  Otherwise (and always for larger R, where a
+    // lane's share of the window is well above half) all lanes step through
+    // every event of the chunk, broadcast with readlane.
+    const unsigned span = unsigned(2 * R);
     for (int base = 0; base < E; base += kWave) {
       const int cnt = min(kWave, E - base);
       const T vz = L < cnt ? sm.ez[base + L] : T(0);
       const int vp = L < cnt ? sm.ep[base + L] : 0;
-      for (int i = 0; i < cnt; ++i) {
-        const int raw = __builtin_amdgcn_readlane(vp, i);
-        const int pk = raw & ~kGroupEnd;
-        const int kind = pk & 3;
-        if (kind == 2)
-          st.clear((pk >> 2) - L);
-        else
-          st.set((pk >> 2) - L);
-        if (raw & kGroupEnd) {
-          const T z = lane_bcast(vz, i);
-          st.emit(z, kind == 1, active, g);
-          if (kind == 1) {
-            for (int t = gstart; t <= base + i; ++t) st.clear(((sm.ep[t] & ~kGroupEnd) >> 2) - L);
-            st.emit(z, false, active, g);
-          }
-          gstart = base + i + 1;
+      bool walk = false;
+      uint64_t mine = 0;
+      if (NW <= 2 && R <= kWalkMaxR) {
+        for (int i = 0; i < cnt; ++i) {
+          const int col = __builtin_amdgcn_readlane(vp, i) >> 2;
+          mine |= unsigned(col - L) <= span ? (uint64_t(1) << i) : 0;
+        }
+        walk = 4 * wave_max(popc(mine)) < 3 * cnt;
+      }
+      if (walk) {
+        for (;;) {
+          const bool on = mine != 0;
+          if (ballot(on) == 0) break;
+          const int i = on ? int(__builtin_ctzll(mine)) : 0;
+          mine &= mine - 1;
+          const int pk = __shfl(vp, i);
+          const T z = __shfl(vz, i);
+          st.apply(z, pk & 3, (pk >> 2) - L, on, active, g);
+        }
+      } else {
+        for (int i = 0; i < cnt; ++i) {
+          const int pk = __builtin_amdgcn_readlane(vp, i);
+          const int k = (pk >> 2) - L;
+          st.apply_all(lane_bcast(vz, i), pk & 3, k, unsigned(k) <= span, active, g);
         }
       }
     }
+    // equal z never straddles a slab (slabs end at bin boundaries)
+    if (ballot(st.pend) != 0) st.close_points(st.pend, active, g);
     b_lo = b_hi;
   }
   if (ballot(active && !st.ok) != 0) raise_status(g.status, kStatusArenaOverflow);

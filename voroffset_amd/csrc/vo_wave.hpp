@@ -24,6 +24,11 @@ __device__ __forceinline__ int count_below(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
 }
 __device__ __forceinline__ int popc(uint64_t m) { return __popcll(m); }
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
 
 // Workgroup b of a launch runs on XCD b % 8 (the dispatcher deals workgroups
 // to the eight XCDs round-robin).  xcd_tile maps b to a logical tile so that
