@@ -1527,21 +1527,23 @@ __device__ __forceinline__ void column_segments(const T* z, int64_t s, int64_t n
   f(ca, cb);
 }
 
-// Sweep state of one lane: coverage bits of the 2R + 1 window columns within
-// reach (bit k <-> window column L + k, so the lane's own column is bit R),
-// held in NW 64-bit registers, the distance to the nearest covered column and
-// the open piece.
-// The sweep only compares endpoint values it read, so the open piece's start
-// is kept in the storage type T.
 // base[i] as a scalar base plus a 32-bit vector byte offset
 template <class V>
 __device__ __forceinline__ void store_at(V* base, uint32_t i, V v) {
   *reinterpret_cast<V*>(reinterpret_cast<char*>(base) + i * uint32_t(sizeof(V))) = v;
 }
 
+// Sweep state of one lane.  The lane's own column is window position L + R;
+// a covered window column L + k within reach (0 <= k <= 2R) at distance
+// d = |k - R| is bit d of `up` (k >= R) or of `dn` (k < R), NW 64-bit words
+// per side (d < 64 NW), so the distance to the nearest covered column is the
+// lowest set bit of up | dn.  Within one column the merged segments neither
+// overlap nor touch, so every start or end event flips its column's bit.
+// The sweep only compares endpoint values it read, so the open piece's start
+// is kept in the storage type T.
 template <int NW, class T>
 struct ZState {
-  uint64_t w[NW];
+  uint64_t up[NW], dn[NW];
   int R = 0;
   int dist = 1 << 20, dcur = 1 << 20, count = 0;
   T z0 = T(0);
@@ -1559,48 +1561,21 @@ struct ZState {
   // the lane's next event that is not a point at the same z)
   bool pend = false;
   T pz = T(0);
-  uint64_t gpts[NW];
+  uint64_t gup[NW], gdn[NW];
 
   __device__ __forceinline__ explicit ZState(int r) : R(r) {
 #pragma unroll
-    for (int t = 0; t < NW; ++t) w[t] = gpts[t] = 0;
+    for (int t = 0; t < NW; ++t) up[t] = dn[t] = gup[t] = gdn[t] = 0;
   }
-  __device__ __forceinline__ uint64_t word(int i) const {
-    uint64_t v = 0;
-#pragma unroll
-    for (int t = 0; t < NW; ++t) v = t == i ? w[t] : v;
-    return v;
-  }
-  // distance from bit R to the nearest set bit, or R + 1 when none is set
+  // distance to the nearest covered column, or R + 1 when none is
   __device__ __forceinline__ int nearest() const {
-    int best = R + 1;
+    const uint64_t m0 = up[0] | dn[0];
     if constexpr (NW == 1) {
-      const uint64_t up = w[0] >> R, dn = w[0] << (63 - R);
-      if (up) best = __builtin_ctzll(up);
-      if (dn) best = min(best, __builtin_clzll(dn));
+      return m0 ? int(__builtin_ctzll(m0)) : R + 1;
     } else {
-      for (int q = R, span = 0; span <= R && q < NW * 64;) {
-        const int b = q & 63;
-        const uint64_t v = word(q >> 6) >> b;
-        if (v) {
-          best = min(best, span + __builtin_ctzll(v));
-          break;
-        }
-        span += 64 - b;
-        q += 64 - b;
-      }
-      for (int q = R, span = 0; span <= R && q >= 0;) {
-        const int b = q & 63;
-        const uint64_t v = word(q >> 6) << (63 - b);
-        if (v) {
-          best = min(best, span + __builtin_clzll(v));
-          break;
-        }
-        span += b + 1;
-        q -= b + 1;
-      }
+      const uint64_t m1 = up[1] | dn[1];
+      return m0 ? int(__builtin_ctzll(m0)) : (m1 ? 64 + int(__builtin_ctzll(m1)) : R + 1);
     }
-    return best;
   }
   // The updates below are written with selects, predicated by `on`: lanes
   // walk their own events in lock step, and only the store of a finished
@@ -1640,74 +1615,38 @@ struct ZState {
   __device__ __forceinline__ void close_points(bool on, bool active, const Pass1ZArgs<T>& g) {
     emit(pz, true, on, active, g);
 #pragma unroll
-    for (int t = 0; t < NW; ++t) w[t] &= on ? ~gpts[t] : ~uint64_t(0);
-    const int nd = nearest();
-    dist = on ? nd : dist;
+    for (int t = 0; t < NW; ++t) {
+      up[t] &= on ? ~gup[t] : ~uint64_t(0);
+      dn[t] &= on ? ~gdn[t] : ~uint64_t(0);
+      gup[t] = on ? 0 : gup[t];
+      gdn[t] = on ? 0 : gdn[t];
+    }
+    dist = nearest();
     emit(pz, false, on, active, g);
-#pragma unroll
-    for (int t = 0; t < NW; ++t) gpts[t] = on ? 0 : gpts[t];
     pend = pend && !on;
   }
-  // one event of a window column within reach (bit k, 0 <= k <= 2R) where
-  // `on`; kind 0 start, 1 point, 2 end
+  // one event of window column L + k within reach (0 <= k <= 2R) where `on`;
+  // kind 0 start, 1 point, 2 end
   __device__ __forceinline__ void apply(T z, int kind, int k, bool on, bool active, const Pass1ZArgs<T>& g) {
     const bool point = kind == 1;
     const bool flush = on && pend && !(point && z == pz);
     if (ballot(flush) != 0) close_points(flush, active, g);
-    const int kk = on ? k : R;
-    const uint64_t b = on ? (uint64_t(1) << (kk & 63)) : 0;
-    const bool end = kind == 2;
+    const bool upper = k >= R;
+    const int d = upper ? k - R : R - k;
+    const uint64_t b = on ? (uint64_t(1) << (d & 63)) : 0;
 #pragma unroll
     for (int t = 0; t < NW; ++t) {
-      const uint64_t bt = (NW == 1 || t == (kk >> 6)) ? b : 0;
-      w[t] = end ? (w[t] & ~bt) : (w[t] | bt);
-      gpts[t] |= point ? bt : 0;
+      const uint64_t bt = (NW == 1 || t == (d >> 6)) ? b : 0;
+      const uint64_t bu = upper ? bt : 0, bd = upper ? 0 : bt;
+      up[t] ^= bu;
+      dn[t] ^= bd;
+      gup[t] |= point ? bu : 0;
+      gdn[t] |= point ? bd : 0;
     }
-    const int dd = kk > R ? kk - R : R - kk;
-    dist = (on && !end && dd < dist) ? dd : dist;
-    const bool again = on && end && dd == dist;  // the cleared column was this lane's nearest
-    if (ballot(again) != 0) {
-      const int nd = nearest();
-      dist = again ? nd : dist;
-    }
+    dist = nearest();
     pz = (on && point) ? z : pz;
     pend = pend || (on && point);
     emit(z, false, on && !point, active, g);
-  }
-  // the same for an event broadcast to every lane: `kind` is uniform, so the
-  // kinds take branches of their own
-  __device__ __forceinline__ void apply_all(T z, int kind, int k, bool on, bool active, const Pass1ZArgs<T>& g) {
-    const bool point = kind == 1;
-    const bool flush = on && pend && !(point && z == pz);
-    if (ballot(flush) != 0) close_points(flush, active, g);
-    const int kk = on ? k : R;
-    const uint64_t b = on ? (uint64_t(1) << (kk & 63)) : 0;
-    const int dd = kk > R ? kk - R : R - kk;
-    if (kind == 2) {
-#pragma unroll
-      for (int t = 0; t < NW; ++t) w[t] &= (NW == 1 || t == (kk >> 6)) ? ~b : ~uint64_t(0);
-      const bool again = on && dd == dist;
-      if constexpr (NW == 1) {
-        const int nd = nearest();  // a few instructions: cheaper than a branch
-        dist = again ? nd : dist;
-      } else if (ballot(again) != 0) {
-        const int nd = nearest();
-        dist = again ? nd : dist;
-      }
-      emit(z, false, on, active, g);
-    } else {
-#pragma unroll
-      for (int t = 0; t < NW; ++t) w[t] |= (NW == 1 || t == (kk >> 6)) ? b : 0;
-      dist = (on && dd < dist) ? dd : dist;
-      if (point) {
-#pragma unroll
-        for (int t = 0; t < NW; ++t) gpts[t] |= (NW == 1 || t == (kk >> 6)) ? b : 0;
-        pz = on ? z : pz;
-        pend = pend || on;
-      } else {
-        emit(z, false, on, active, g);
-      }
-    }
   }
 };
 
@@ -1800,7 +1739,7 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
     return;
   }
   // per-lane sweep state (lane = output column y0 + L, window position L + R)
-  if (2 * R + 1 > NW * 64) {  // the host picks NW from R; never taken
+  if (R >= NW * 64) {  // the host picks NW from R; never taken
     raise_status(g.status, kStatusP1List);
     return;
   }
@@ -1900,7 +1839,7 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
       const int vp = L < cnt ? sm.ep[base + L] : 0;
       bool walk = false;
       uint64_t mine = 0;
-      if (NW <= 2 && R <= kWalkMaxR) {
+      if (R <= kWalkMaxR) {
         for (int i = 0; i < cnt; ++i) {
           const int col = __builtin_amdgcn_readlane(vp, i) >> 2;
           mine |= unsigned(col - L) <= span ? (uint64_t(1) << i) : 0;
@@ -1921,7 +1860,7 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
         for (int i = 0; i < cnt; ++i) {
           const int pk = __builtin_amdgcn_readlane(vp, i);
           const int k = (pk >> 2) - L;
-          st.apply_all(lane_bcast(vz, i), pk & 3, k, unsigned(k) <= span, active, g);
+          st.apply(lane_bcast(vz, i), pk & 3, k, unsigned(k) <= span, active, g);
         }
       }
     }
