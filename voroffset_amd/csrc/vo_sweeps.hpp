@@ -1075,15 +1075,22 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
   // slots; values past the column's count are read and discarded): dilated
   // and fused into the open run, which is united when a gap opens.  Loads are
   // unconditional and the loop is written with selects, so it has no branches.
-  auto process_block = [&](int c, int jb, int m, const double* hrow, bool& run, double& rs, double& re) {
-    const int64_t o = slot_index(g.ny, g.caps, c, yc, jb);
+  struct Block {
     T va[B], vb[B];
+    uint4 q;  // the B 16-bit line offsets
+  };
+  auto load_item = [&](int c, int jb, Block& bk) {
+    const int64_t o = slot_index(g.ny, g.caps, c, yc, jb);
+    load_block(g.m_lo + o, bk.va);
+    load_block(g.m_hi + o, bk.vb);
+    bk.q = *reinterpret_cast<const uint4*>(g.m_dy + o);
+  };
+  auto compute_block = [&](const Block& bk, int jb, int m, const double* hrow, bool& run, double& rs, double& re) {
+    const T* va = bk.va;
+    const T* vb = bk.vb;
     int vdy[B];
-    load_block(g.m_lo + o, va);
-    load_block(g.m_hi + o, vb);
     {
-      const uint4 q = *reinterpret_cast<const uint4*>(g.m_dy + o);
-      const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+      const uint32_t w[4] = {bk.q.x, bk.q.y, bk.q.z, bk.q.w};
 #pragma unroll
       for (int u = 0; u < B; ++u) vdy[u] = int((w[u >> 1] >> (16 * (u & 1))) & 0xffffu);
     }
@@ -1101,30 +1108,66 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
       run = run || valid;
     }
   };
+  auto process_block = [&](int c, int jb, int m, const double* hrow, bool& run, double& rs, double& re) {
+    Block bk;
+    load_item(c, jb, bk);
+    compute_block(bk, jb, m, hrow, run, rs, re);
+  };
   if (g.s_lo) {
     // Source columns nearest first (d = 0, -1, +1, -2, ...), their blocks in
     // z order.  The near columns build most of the union; a farther block
     // whose summary range, widened by the largest dilation at that distance
     // H(d, 0), lies inside every lane's cached interval changes nothing and
     // is neither loaded nor processed.
-    for (int step = 0; step <= 2 * rint; ++step) {
-      const int ad = (step + 1) >> 1;
-      const int d = (step & 1) ? -ad : ad;
-      const int c = x + d;
-      if (c < 0 || c >= g.nx) continue;  // uniform
-      const int m = count_of(c);
-      const double* hrow = htab + ad * (rint + 1);
-      const double hmax = hrow[0];
-      bool run = false;
-      double rs = 0, re = 0;
-      for (int jb = 0; ballot(m > jb && ok) != 0; jb += B) {
-        const int64_t si = summary_index(g.ny, g.caps, c, yc, jb);
-        const double blo = double(g.s_lo[si]), bhi = double(g.s_hi[si]);
-        const bool inside = jb >= m || !ok || (blo - hmax >= clo && bhi + hmax <= chi);
-        if (ballot(!inside) == 0) continue;
-        process_block(c, jb, m, hrow, run, rs, re);
+    //
+    // The items (column, block) are walked with the next item's summary and
+    // the next column's count loaded while the current block is processed,
+    // so that one memory round trip (the block itself) is exposed per item.
+    auto col_of_step = [&](int step) { return x + ((step & 1) ? -((step + 1) >> 1) : ((step + 1) >> 1)); };
+    auto next_step = [&](int step) {  // next step whose column is inside the grid, or 2R + 1
+      for (++step; step <= 2 * rint; ++step) {
+        const int c = col_of_step(step);
+        if (c >= 0 && c < g.nx) break;
       }
-      unite(run && ok, rs, re);
+      return step;
+    };
+    const int last = 2 * rint;
+    int step = next_step(-1);  // step 0 (d = 0) is always inside
+    int c = col_of_step(step);
+    int m = count_of(c);
+    int step2 = next_step(step);
+    int m2 = step2 <= last ? count_of(col_of_step(step2)) : 0;
+    int jb = 0;
+    T blo = g.s_lo[summary_index(g.ny, g.caps, c, yc, 0)];
+    T bhi = g.s_hi[summary_index(g.ny, g.caps, c, yc, 0)];
+    bool run = false;
+    double rs = 0, re = 0;
+    for (;;) {
+      const bool more = ballot(m > jb + B && ok) != 0;  // another block of this column
+      const int nc = more ? c : (step2 <= last ? col_of_step(step2) : c);
+      const int njb = more ? jb + B : 0;
+      const T nblo = g.s_lo[summary_index(g.ny, g.caps, nc, yc, njb)];
+      const T nbhi = g.s_hi[summary_index(g.ny, g.caps, nc, yc, njb)];
+      if (ballot(m > jb && ok) != 0) {
+        const int ad = (step + 1) >> 1;
+        const double* hrow = htab + ad * (rint + 1);
+        const double hmax = hrow[0];
+        const bool inside = jb >= m || !ok || (double(blo) - hmax >= clo && double(bhi) + hmax <= chi);
+        if (ballot(!inside) != 0) process_block(c, jb, m, hrow, run, rs, re);
+      }
+      if (!more) {
+        unite(run && ok, rs, re);
+        run = false;
+        if (step2 > last) break;
+        step = step2;
+        c = nc;
+        m = m2;
+        step2 = next_step(step);
+        m2 = step2 <= last ? count_of(col_of_step(step2)) : 0;
+      }
+      jb = njb;
+      blo = nblo;
+      bhi = nbhi;
     }
   } else {
     // Without summaries: blocks outermost, source columns nearest first
