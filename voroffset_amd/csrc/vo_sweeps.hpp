@@ -1210,14 +1210,21 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
   g.o_cnt[col] = 2 * n;
 }
 
+// Waves per SIMD the gather is compiled for: the kernel waits on memory most
+// of the time, so occupancy pays until the registers spill.
+template <class T>
+struct GatherWaves {
+  static constexpr int value = sizeof(T) == 4 ? 6 : 4;
+};
+
 template <class T, bool kLdsTable>
-__global__ __launch_bounds__(kWave) void pass2_gather_kernel(Pass2GArgs<T> g) {
+__global__ __launch_bounds__(kWave, GatherWaves<T>::value) void pass2_gather_kernel(Pass2GArgs<T> g) {
   gather_tile<T, kLdsTable>(g, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // Batched: job j owns the global tiles [tile_base[j], tile_base[j + 1]).
 template <class T, bool kLdsTable>
-__global__ __launch_bounds__(kWave) void pass2_gather_batch_kernel(const Pass2GArgs<T>* jobs,
+__global__ __launch_bounds__(kWave, GatherWaves<T>::value) void pass2_gather_batch_kernel(const Pass2GArgs<T>* jobs,
                                                                    const int64_t* tile_base, int njobs) {
   const int64_t t = xcd_tile(blockIdx.x, gridDim.x);
   const int j = find_job(tile_base, njobs, t);
