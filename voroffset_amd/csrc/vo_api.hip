@@ -92,7 +92,7 @@ enum Slot {
   kB_TAB0, kB_TAB1, kB_TAB2, kB_STATUS, kB_FIN_OFF, kB_FIN_AUX,         // batched form
   kG_LISTS, kG_HTAB,                                         // gather interval lists, H table
   kM_PAIRS, kM_SEGOFF, kM_VERT, kM_QUAD,                     // dexel mesh writer
-  kM_SLO, kM_SHI,                                            // pass-1 block summaries
+  kM_SLO, kM_SHI, kM_CHI,                                    // pass-1 block and column summaries
   kX_SCR, kX_N1, kX_N2, kX_LEN, kX_OFF, kX_Z, kX_LENGTH, kX_SEGS, kX_SUM,  // parity metrics
   kB_OUT,                                  // + (stage * 3 + kind) * 2 (+1 for z): stage outputs
   kSLOTS = kB_OUT + 4 * 3 * 2
@@ -318,6 +318,7 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
     uint16_t* m_dy = static_cast<uint16_t*>(ws(ctx, kM_DY, slots * sizeof(uint16_t)));
     T* s_lo = static_cast<T*>(ws(ctx, kM_SLO, size_t(slots / vo::kSlotBlock) * sizeof(T)));
     T* s_hi = static_cast<T*>(ws(ctx, kM_SHI, size_t(slots / vo::kSlotBlock) * sizeof(T)));
+    T* c_hi = static_cast<T*>(ws(ctx, kM_CHI, size_t(ncols) * sizeof(T)));
     // pass 2: gather tiles of (x, 64 consecutive y), and the sweep form,
     // either over the whole volume in chunks of chunk2 columns of a row, or
     // (list mode) over the gather tiles that overflowed, one column a block
@@ -347,7 +348,7 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
                         vo::zsweep_offsets_fit(caps, ny, int(sizeof(T)));
     if (zsweep) {
       vo::Pass1ZArgs<T> az{nx, ny, rint, in.off, static_cast<const T*>(in.z), m_lo, m_hi, m_dy, caps, m_cnt,
-                           status, s_lo, s_hi};
+                           status, s_lo, s_hi, c_hi};
       const int64_t gridz = int64_t(nx) * ((ny + vo::kZTile - 1) / vo::kZTile);
       launch_zsweep_events<T>(ctx, events_for(ctx, rint), rint, gridz, az);
     } else if (ctx->pass1_form == 2) {
@@ -367,7 +368,7 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
             static_cast<double*>(ws(ctx, kG_LISTS, size_t(ntiles) * 2 * gocap * vo::kWave * sizeof(double)));
         // the summaries exist only when the z-sweep form wrote pass 1
         vo::Pass2GArgs<T> ag{nx, ny, rint, R2, m_lo, m_hi, m_dy, caps, m_cnt, o_z, garena, o_start, o_len,
-                             tiles, status, gocap, ht, 0, lists, zsweep ? s_lo : nullptr, zsweep ? s_hi : nullptr};
+                             tiles, status, gocap, ht, 0, lists, zsweep ? s_lo : nullptr, zsweep ? s_hi : nullptr, c_hi};
         if (rint <= vo::kGatherLdsTableMax)
           hipLaunchKernelGGL((vo::pass2_gather_kernel<T, true>), dim3(unsigned(ntiles)), dim3(vo::kWave),
                              size_t(rint + 1) * (rint + 1) * sizeof(double), ctx->stream, ag);
@@ -585,6 +586,7 @@ void dilate_batch(vo_context* ctx, std::vector<DevVol>& vol, const std::vector<d
     uint16_t* m_dy = static_cast<uint16_t*>(ws(ctx, kM_DY, size_t(C) * caps * sizeof(uint16_t)));
     T* s_lo = static_cast<T*>(ws(ctx, kM_SLO, size_t(C) * (caps / vo::kSlotBlock) * sizeof(T)));
     T* s_hi = static_cast<T*>(ws(ctx, kM_SHI, size_t(C) * (caps / vo::kSlotBlock) * sizeof(T)));
+    T* c_hi = static_cast<T*>(ws(ctx, kM_CHI, size_t(C) * sizeof(T)));
     int32_t* m_cnt = static_cast<int32_t*>(ws(ctx, kM_CNT, size_t(C) * sizeof(int32_t)));
     T* o_z = static_cast<T*>(ws(ctx, kO_Z, size_t(NT) * garena * sizeof(T)));
     int64_t* o_start = static_cast<int64_t*>(ws(ctx, kO_START, size_t(C) * sizeof(int64_t)));
@@ -600,11 +602,11 @@ void dilate_batch(vo_context* ctx, std::vector<DevVol>& vol, const std::vector<d
       const int64_t cb = col_base[k], sb = cb * caps;
       const int64_t qb = cb * (caps / vo::kSlotBlock);  // first block summary of this volume
       a1[k] = vo::Pass1ZArgs<T>{v.nx, v.ny, rint, v.off, static_cast<const T*>(v.z), m_lo + sb, m_hi + sb, m_dy + sb,
-                                caps, m_cnt + cb, status + k, s_lo + qb, s_hi + qb};
+                                caps, m_cnt + cb, status + k, s_lo + qb, s_hi + qb, c_hi + cb};
       a2[k] = vo::Pass2GArgs<T>{v.nx, v.ny, rint, R[jobs[k]] * R[jobs[k]], m_lo + sb, m_hi + sb, m_dy + sb, caps,
                                 m_cnt + cb, o_z, garena, o_start + cb, o_len + cb, nullptr, status + k, ocap,
                                 htab + htab_base[k], tile_base[k] * garena, lists + tile_base[k] * list_tile,
-                                s_lo + qb, s_hi + qb};
+                                s_lo + qb, s_hi + qb, c_hi + cb};
     }
     const auto* d1 = upload_table(ctx, kB_TAB0, a1);
     const auto* d2 = upload_table(ctx, kB_TAB1, a2);

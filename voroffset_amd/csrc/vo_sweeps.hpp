@@ -934,6 +934,7 @@ struct Pass2GArgs {
   double* lists;        // interval lists, 2 * ocap * 64 doubles per tile (lane-minor)
   const T* s_lo;        // block summaries from pass 1 (null: none, no block is skipped)
   const T* s_hi;
+  const T* c_hi;        // per column: hi of its last triple (with the summaries)
 };
 
 
@@ -990,6 +991,9 @@ __device__ __forceinline__ void load_block(const double* p, double (&v)[kSlotBlo
     v[i + 1] = a.y;
   }
 }
+
+// Source columns whose ranges the gather checks per memory round trip.
+constexpr int kGatherGroup = 4;
 
 // Largest radius whose H table is staged in LDS ((R + 1)^2 doubles, 8 KB);
 // above it the lanes read the table from global memory (L1-resident).
@@ -1115,59 +1119,77 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
   };
   if (g.s_lo) {
     // Source columns nearest first (d = 0, -1, +1, -2, ...), their blocks in
-    // z order.  The near columns build most of the union; a farther block
-    // whose summary range, widened by the largest dilation at that distance
-    // H(d, 0), lies inside every lane's cached interval changes nothing and
-    // is neither loaded nor processed.
+    // z order.  The near columns build most of the union; a farther column
+    // or block whose range, widened by the largest dilation at that distance
+    // H(d, 0), lies inside every lane's cached interval changes nothing (the
+    // union only grows) and is neither loaded nor processed.
     //
-    // The items (column, block) are walked with the next item's summary and
-    // the next column's count loaded while the current block is processed,
-    // so that one memory round trip (the block itself) is exposed per item.
+    // Columns are checked in groups of kGatherGroup: a group's counts and
+    // column ranges (first lo from the block summaries, last hi) are loaded
+    // while the group before is processed, so the wavefront waits on memory
+    // about once per group of skipped columns.  A column that is processed
+    // has its count and first summary loaded again (one round trip), and the
+    // next block's summary is loaded while the current block is processed.
+    constexpr int G = kGatherGroup;
+    const int nsteps = 2 * rint + 1;
     auto col_of_step = [&](int step) { return x + ((step & 1) ? -((step + 1) >> 1) : ((step + 1) >> 1)); };
-    auto next_step = [&](int step) {  // next step whose column is inside the grid, or 2R + 1
-      for (++step; step <= 2 * rint; ++step) {
+    int pm[G];
+    T plo[G], phi[G];
+    auto fetch = [&](int grp) {
+#pragma unroll
+      for (int j = 0; j < G; ++j) {
+        const int step = grp * G + j;
         const int c = col_of_step(step);
-        if (c >= 0 && c < g.nx) break;
+        pm[j] = 0;
+        plo[j] = phi[j] = T(0);
+        if (step < nsteps && c >= 0 && c < g.nx) {  // uniform
+          pm[j] = count_of(c);
+          plo[j] = g.s_lo[summary_index(g.ny, g.caps, c, yc, 0)];
+          phi[j] = g.c_hi[count_index(g.ny, c, yc)];
+        }
       }
-      return step;
     };
-    const int last = 2 * rint;
-    int step = next_step(-1);  // step 0 (d = 0) is always inside
-    int c = col_of_step(step);
-    int m = count_of(c);
-    int step2 = next_step(step);
-    int m2 = step2 <= last ? count_of(col_of_step(step2)) : 0;
-    int jb = 0;
-    T blo = g.s_lo[summary_index(g.ny, g.caps, c, yc, 0)];
-    T bhi = g.s_hi[summary_index(g.ny, g.caps, c, yc, 0)];
-    bool run = false;
-    double rs = 0, re = 0;
-    for (;;) {
-      const bool more = ballot(m > jb + B && ok) != 0;  // another block of this column
-      const int nc = more ? c : (step2 <= last ? col_of_step(step2) : c);
-      const int njb = more ? jb + B : 0;
-      const T nblo = g.s_lo[summary_index(g.ny, g.caps, nc, yc, njb)];
-      const T nbhi = g.s_hi[summary_index(g.ny, g.caps, nc, yc, njb)];
-      if (ballot(m > jb && ok) != 0) {
+    fetch(0);
+    for (int grp = 0; grp * G < nsteps; ++grp) {
+      unsigned need = 0;
+#pragma unroll
+      for (int j = 0; j < G; ++j) {
+        const int ad = (grp * G + j + 1) >> 1;
+        const double hmax = ad <= rint ? htab[ad * (rint + 1)] : 0.0;
+        const bool in = pm[j] == 0 || !ok || (double(plo[j]) - hmax >= clo && double(phi[j]) + hmax <= chi);
+        need |= ballot(!in) != 0 ? 1u << j : 0u;
+      }
+      if ((grp + 1) * G < nsteps) fetch(grp + 1);
+      while (need != 0) {
+        const int j = __builtin_ctz(need);
+        need &= need - 1;
+        const int step = grp * G + j;
+        const int c = col_of_step(step);
+        const int m = count_of(c);  // loaded with the first summary
         const int ad = (step + 1) >> 1;
         const double* hrow = htab + ad * (rint + 1);
         const double hmax = hrow[0];
-        const bool inside = jb >= m || !ok || (double(blo) - hmax >= clo && double(bhi) + hmax <= chi);
-        if (ballot(!inside) != 0) process_block(c, jb, m, hrow, run, rs, re);
-      }
-      if (!more) {
+        bool run = false;
+        double rs = 0, re = 0;
+        int jb = 0;
+        T blo = g.s_lo[summary_index(g.ny, g.caps, c, yc, 0)];
+        T bhi = g.s_hi[summary_index(g.ny, g.caps, c, yc, 0)];
+        for (;;) {
+          const bool more = ballot(m > jb + B && ok) != 0;
+          T nblo = blo, nbhi = bhi;
+          if (more) {
+            nblo = g.s_lo[summary_index(g.ny, g.caps, c, yc, jb + B)];
+            nbhi = g.s_hi[summary_index(g.ny, g.caps, c, yc, jb + B)];
+          }
+          const bool inside = jb >= m || !ok || (double(blo) - hmax >= clo && double(bhi) + hmax <= chi);
+          if (ballot(!inside) != 0) process_block(c, jb, m, hrow, run, rs, re);
+          if (!more) break;
+          jb += B;
+          blo = nblo;
+          bhi = nbhi;
+        }
         unite(run && ok, rs, re);
-        run = false;
-        if (step2 > last) break;
-        step = step2;
-        c = nc;
-        m = m2;
-        step2 = next_step(step);
-        m2 = step2 <= last ? count_of(col_of_step(step2)) : 0;
       }
-      jb = njb;
-      blo = nblo;
-      bhi = nbhi;
     }
   } else {
     // Without summaries: blocks outermost, source columns nearest first
@@ -1214,7 +1236,7 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
 // of the time, so occupancy pays until the registers spill.
 template <class T>
 struct GatherWaves {
-  static constexpr int value = sizeof(T) == 4 ? 6 : 4;
+  static constexpr int value = sizeof(T) == 4 ? 5 : 4;
 };
 
 template <class T, bool kLdsTable>
@@ -1536,6 +1558,7 @@ struct Pass1ZArgs {
   // its last one
   T* s_lo;
   T* s_hi;
+  T* c_hi;  // per column, at count_index: hi of its last triple (when it has one)
 };
 
 constexpr int kZTile = kWave;    // output columns per wavefront
@@ -1920,6 +1943,10 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
   }
   if (ballot(active && !st.ok) != 0) raise_status(g.status, kStatusArenaOverflow);
   if (active) g.m_cnt[count_index(g.ny, x, y0 + L)] = st.count;
+  if (active && st.count > 0) {  // the lane's own last summary, read back
+    const uint32_t last = uint32_t(min(st.count, g.caps) - 1);
+    g.c_hi[count_index(g.ny, x, y0 + L)] = st.shi_w[st.sidx0 + (last / kSlotBlock) * st.ny];
+  }
 }
 
 template <class T, int ECAP, int NB, int WMAX, int NW>
