@@ -1040,7 +1040,7 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
   // Interval k of the list is cached in registers as [clo, chi], with the
   // bounds of its neighbours (below = hi of k - 1, above = lo of k + 1): a run
   // that meets it without reaching a neighbour is united in registers, and
-  // only the other runs go through the LDS list.
+  // only the other runs go through the list in memory.
   int k = 0;
   double clo = INFINITY, chi = -INFINITY, below = -INFINITY, above = INFINITY;
   bool dirty = false;
@@ -1060,7 +1060,7 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
     above = k + 1 < n ? olo[(k + 1) * kWave] : INFINITY;
   };
   // unites [s, e] where `want` holds; written with selects so that the
-  // wavefront branches only when some lane needs the LDS list
+  // wavefront branches only when some lane needs the list
   auto unite = [&](bool want, double s, double e) {
     if (ballot(want) == 0) return;  // the usual case inside a run
     const bool fast = s <= chi && e >= clo && s > below && e < above;
@@ -1077,8 +1077,8 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
   // The lane's block of B contiguous triples of column c from jb on (jb is a
   // multiple of B and below caps, so the whole block is inside the column's
   // slots; values past the column's count are read and discarded): dilated
-  // and fused into the open run, which is united when a gap opens.  Loads are
-  // unconditional and the loop is written with selects, so it has no branches.
+  // and fused into the open run, which is united when a gap opens.  The loop
+  // is written with selects, so it has no branches.
   struct Block {
     T va[B], vb[B];
     uint4 q;  // the B 16-bit line offsets
@@ -1112,10 +1112,13 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
       run = run || valid;
     }
   };
-  auto process_block = [&](int c, int jb, int m, const double* hrow, bool& run, double& rs, double& re) {
-    Block bk;
-    load_item(c, jb, bk);
-    compute_block(bk, jb, m, hrow, run, rs, re);
+  // lanes that do not need the block (its range is inside their cached
+  // interval) neither load nor use it
+  auto process_block = [&](int c, int jb, int m, const double* hrow, bool& run, double& rs, double& re,
+                           bool need) {
+    Block bk = {};
+    if (need) load_item(c, jb, bk);
+    compute_block(bk, jb, need ? m : jb, hrow, run, rs, re);
   };
   if (g.s_lo) {
     // Source columns nearest first (d = 0, -1, +1, -2, ...), their blocks in
@@ -1182,7 +1185,7 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
             nbhi = g.s_hi[summary_index(g.ny, g.caps, c, yc, jb + B)];
           }
           const bool inside = jb >= m || !ok || (double(blo) - hmax >= clo && double(bhi) + hmax <= chi);
-          if (ballot(!inside) != 0) process_block(c, jb, m, hrow, run, rs, re);
+          if (ballot(!inside) != 0) process_block(c, jb, m, hrow, run, rs, re, !inside);
           if (!more) break;
           jb += B;
           blo = nblo;
@@ -1205,7 +1208,7 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
         if (ballot(m > jb && ok) == 0) continue;
         bool run = false;
         double rs = 0, re = 0;
-        process_block(c, jb, m, htab + ad * (rint + 1), run, rs, re);
+        process_block(c, jb, m, htab + ad * (rint + 1), run, rs, re, true);
         unite(run && ok, rs, re);
       }
     }
@@ -1233,10 +1236,11 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
 }
 
 // Waves per SIMD the gather is compiled for: the kernel waits on memory most
-// of the time, so occupancy pays until the registers spill.
+// of the time, so occupancy pays until the registers spill (at 5 waves the
+// FP32 instantiation spills and runs slower).
 template <class T>
 struct GatherWaves {
-  static constexpr int value = sizeof(T) == 4 ? 5 : 4;
+  static constexpr int value = 4;
 };
 
 template <class T, bool kLdsTable>
