@@ -225,15 +225,17 @@ const double* upload_htab(vo_context* ctx, int slot, const std::vector<double>& 
   return d;
 }
 
-// The z-sweep form of pass 1 with coverage bits for distances 0 .. R per side
-// of a lane (one 64-bit word per side up to R = 63, two up to 127).
+// The z-sweep form of pass 1 with 2R + 2 coverage bits per lane (one 64-bit
+// word up to R = 31, two up to 63, four up to 127).
 template <class T, int ECAP, int NB>
 void launch_zsweep(vo_context* ctx, int rint, int64_t grid, const vo::Pass1ZArgs<T>& az) {
   const dim3 g(static_cast<unsigned>(grid)), b(static_cast<unsigned>(vo::kWave));
-  if (rint <= 63)
+  if (rint <= 31)
     hipLaunchKernelGGL((vo::pass1_zsweep_kernel<T, ECAP, NB, kZWMax, 1>), g, b, 0, ctx->stream, az);
-  else
+  else if (rint <= 63)
     hipLaunchKernelGGL((vo::pass1_zsweep_kernel<T, ECAP, NB, kZWMax, 2>), g, b, 0, ctx->stream, az);
+  else
+    hipLaunchKernelGGL((vo::pass1_zsweep_kernel<T, ECAP, NB, kZWMax, 4>), g, b, 0, ctx->stream, az);
 }
 
 template <class T>
@@ -251,11 +253,14 @@ template <class T, int ECAP, int NB>
 void launch_zsweep_batch(vo_context* ctx, int max_rint, int64_t grid, const vo::Pass1ZArgs<T>* jobs,
                          const int64_t* tile_base, int njobs) {
   const dim3 g(static_cast<unsigned>(grid)), b(static_cast<unsigned>(vo::kWave));
-  if (max_rint <= 63)
+  if (max_rint <= 31)
     hipLaunchKernelGGL((vo::pass1_zsweep_batch_kernel<T, ECAP, NB, kZWMax, 1>), g, b, 0, ctx->stream, jobs, tile_base,
                        njobs);
-  else
+  else if (max_rint <= 63)
     hipLaunchKernelGGL((vo::pass1_zsweep_batch_kernel<T, ECAP, NB, kZWMax, 2>), g, b, 0, ctx->stream, jobs, tile_base,
+                       njobs);
+  else
+    hipLaunchKernelGGL((vo::pass1_zsweep_batch_kernel<T, ECAP, NB, kZWMax, 4>), g, b, 0, ctx->stream, jobs, tile_base,
                        njobs);
 }
 

@@ -1611,16 +1611,20 @@ __device__ __forceinline__ void store_at(V* base, uint32_t i, V v) {
 }
 
 // Sweep state of one lane.  The lane's own column is window position L + R;
-// a covered window column L + k within reach (0 <= k <= 2R) at distance
-// d = |k - R| is bit d of `up` (k >= R) or of `dn` (k < R), NW 64-bit words
-// per side (d < 64 NW), so the distance to the nearest covered column is the
-// lowest set bit of up | dn.  Within one column the merged segments neither
-// overlap nor touch, so every start or end event flips its column's bit.
+// a covered window column L + k within reach (0 <= k <= 2R) is kept as one
+// bit indexed by its distance d = |k - R|, so the distance to the nearest
+// covered column is one bit scan.  With NW = 1 (R <= 31) the sides
+// interleave in one 64-bit word: bit 2d for k >= R, bit 2d + 1 for k < R, and
+// the distance is half the lowest set bit's index.  With NW = 2 or 4 each side
+// has NW / 2 words of its own (bit d of the side's words; R < 32 NW) and the
+// distance is the lowest set bit of their union.  Within one column the
+// merged segments neither overlap nor touch, so every start or end event
+// flips its column's bit.
 // The sweep only compares endpoint values it read, so the open piece's start
 // is kept in the storage type T.
 template <int NW, class T>
 struct ZState {
-  uint64_t up[NW], dn[NW];
+  uint64_t w[NW];
   int R = 0;
   int dist = 1 << 20, dcur = 1 << 20, count = 0;
   T z0 = T(0);
@@ -1638,20 +1642,42 @@ struct ZState {
   // the lane's next event that is not a point at the same z)
   bool pend = false;
   T pz = T(0);
-  uint64_t gup[NW], gdn[NW];
+  uint64_t gpts[NW];
 
   __device__ __forceinline__ explicit ZState(int r) : R(r) {
 #pragma unroll
-    for (int t = 0; t < NW; ++t) up[t] = dn[t] = gup[t] = gdn[t] = 0;
+    for (int t = 0; t < NW; ++t) w[t] = gpts[t] = 0;
   }
+  static constexpr int H = NW == 1 ? 1 : NW / 2;  // words per side (NW > 1)
   // distance to the nearest covered column, or R + 1 when none is
   __device__ __forceinline__ int nearest() const {
-    const uint64_t m0 = up[0] | dn[0];
     if constexpr (NW == 1) {
-      return m0 ? int(__builtin_ctzll(m0)) : R + 1;
+      return w[0] ? int(__builtin_ctzll(w[0])) >> 1 : R + 1;
     } else {
-      const uint64_t m1 = up[1] | dn[1];
-      return m0 ? int(__builtin_ctzll(m0)) : (m1 ? 64 + int(__builtin_ctzll(m1)) : R + 1);
+      int best = R + 1;
+#pragma unroll
+      for (int t = H - 1; t >= 0; --t) {
+        const uint64_t m = w[t] | w[H + t];
+        best = m ? 64 * t + int(__builtin_ctzll(m)) : best;
+      }
+      return best;
+    }
+  }
+  // the bits of window column k (0 <= k <= 2R) where `on`, word by word
+  __device__ __forceinline__ void column_bits(int k, bool on, uint64_t (&bits)[NW]) const {
+    if constexpr (NW == 1) {
+      const int idx = k >= R ? 2 * (k - R) : 2 * (R - k) + 1;
+      bits[0] = on ? (uint64_t(1) << idx) : 0;
+    } else {
+      const bool upper = k >= R;
+      const int d = upper ? k - R : R - k;
+      const uint64_t b = on ? (uint64_t(1) << (d & 63)) : 0;
+#pragma unroll
+      for (int t = 0; t < H; ++t) {
+        const uint64_t bt = (H == 1 || t == (d >> 6)) ? b : 0;
+        bits[t] = upper ? bt : 0;
+        bits[H + t] = upper ? 0 : bt;
+      }
     }
   }
   // The updates below are written with selects, predicated by `on`: lanes
@@ -1693,10 +1719,8 @@ struct ZState {
     emit(pz, true, on, active, g);
 #pragma unroll
     for (int t = 0; t < NW; ++t) {
-      up[t] &= on ? ~gup[t] : ~uint64_t(0);
-      dn[t] &= on ? ~gdn[t] : ~uint64_t(0);
-      gup[t] = on ? 0 : gup[t];
-      gdn[t] = on ? 0 : gdn[t];
+      w[t] &= on ? ~gpts[t] : ~uint64_t(0);
+      gpts[t] = on ? 0 : gpts[t];
     }
     dist = nearest();
     emit(pz, false, on, active, g);
@@ -1708,17 +1732,12 @@ struct ZState {
     const bool point = kind == 1;
     const bool flush = on && pend && !(point && z == pz);
     if (ballot(flush) != 0) close_points(flush, active, g);
-    const bool upper = k >= R;
-    const int d = upper ? k - R : R - k;
-    const uint64_t b = on ? (uint64_t(1) << (d & 63)) : 0;
+    uint64_t bits[NW];
+    column_bits(k, on, bits);
 #pragma unroll
     for (int t = 0; t < NW; ++t) {
-      const uint64_t bt = (NW == 1 || t == (d >> 6)) ? b : 0;
-      const uint64_t bu = upper ? bt : 0, bd = upper ? 0 : bt;
-      up[t] ^= bu;
-      dn[t] ^= bd;
-      gup[t] |= point ? bu : 0;
-      gdn[t] |= point ? bd : 0;
+      w[t] ^= bits[t];
+      gpts[t] |= point ? bits[t] : 0;
     }
     dist = nearest();
     pz = (on && point) ? z : pz;
@@ -1816,7 +1835,7 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
     return;
   }
   // per-lane sweep state (lane = output column y0 + L, window position L + R)
-  if (R >= NW * 64) {  // the host picks NW from R; never taken
+  if (NW == 1 ? 2 * R + 1 >= 64 : R >= 32 * NW) {  // the host picks NW from R; never taken
     raise_status(g.status, kStatusP1List);
     return;
   }
