@@ -1628,7 +1628,9 @@ struct ZState {
   int R = 0;
   int dist = 1 << 20, dcur = 1 << 20, count = 0;
   T z0 = T(0);
-  bool point_open = false, ok = true;
+  // loop-carried flags as 0/1 integers: in vector registers they need no
+  // exec-mask merges at every join of the walk
+  int point_open = 0, ok = 1;
   // the wavefront's first slot and summary in each array (uniform), and this
   // lane's slot and summary offsets from them for triple 0
   T* lo_w = nullptr;
@@ -1640,7 +1642,7 @@ struct ZState {
   uint32_t ny = 0;
   // the lane's pending group of points of equal z (their bits stay set until
   // the lane's next event that is not a point at the same z)
-  bool pend = false;
+  int pend = 0;
   T pz = T(0);
   uint64_t gpts[NW];
 
@@ -1694,7 +1696,7 @@ struct ZState {
   // zero-length pieces are kept only when a point opened them
   __device__ __forceinline__ void emit(T z, bool is_point, bool on, bool active, const Pass1ZArgs<T>& g) {
     const bool changed = on && dist != dcur;
-    const bool put = changed && dcur <= R && (z0 < z || point_open) && active;
+    const bool put = changed && dcur <= R && (z0 < z || point_open != 0) && active;
     if (put && count < g.caps) {
       const uint32_t j = uint32_t(count);
       const uint32_t blk = j / kSlotBlock;
@@ -1706,11 +1708,11 @@ struct ZState {
       if (j % kSlotBlock == 0) store_at(slo_w, q, z0);
       store_at(shi_w, q, z);
     }
-    ok = ok && !(put && count >= g.caps);
+    ok = (put && count >= g.caps) ? 0 : ok;
     count += put ? 1 : 0;
     const bool open = changed && dist <= R;
     z0 = open ? z : z0;
-    point_open = open ? is_point : point_open;
+    point_open = open ? int(is_point) : point_open;
     dcur = on ? dist : dcur;
   }
   // a finished point group: pieces are closed at its z with the points in
@@ -1724,13 +1726,13 @@ struct ZState {
     }
     dist = nearest();
     emit(pz, false, on, active, g);
-    pend = pend && !on;
+    pend = on ? 0 : pend;
   }
   // one event of window column L + k within reach (0 <= k <= 2R) where `on`;
   // kind 0 start, 1 point, 2 end
   __device__ __forceinline__ void apply(T z, int kind, int k, bool on, bool active, const Pass1ZArgs<T>& g) {
     const bool point = kind == 1;
-    const bool flush = on && pend && !(point && z == pz);
+    const bool flush = on && pend != 0 && !(point && z == pz);
     if (ballot(flush) != 0) close_points(flush, active, g);
     uint64_t bits[NW];
     column_bits(k, on, bits);
@@ -1741,7 +1743,7 @@ struct ZState {
     }
     dist = nearest();
     pz = (on && point) ? z : pz;
-    pend = pend || (on && point);
+    pend = (on && point) ? 1 : pend;
     emit(z, false, on && !point, active, g);
   }
 };
@@ -1961,10 +1963,10 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
       }
     }
     // equal z never straddles a slab (slabs end at bin boundaries)
-    if (ballot(st.pend) != 0) st.close_points(st.pend, active, g);
+    if (ballot(st.pend != 0) != 0) st.close_points(st.pend != 0, active, g);
     b_lo = b_hi;
   }
-  if (ballot(active && !st.ok) != 0) raise_status(g.status, kStatusArenaOverflow);
+  if (ballot(active && st.ok == 0) != 0) raise_status(g.status, kStatusArenaOverflow);
   if (active) g.m_cnt[count_index(g.ny, x, y0 + L)] = st.count;
   if (active && st.count > 0) {  // the lane's own last summary, read back
     const uint32_t last = uint32_t(min(st.count, g.caps) - 1);
