@@ -1036,22 +1036,24 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
 #pragma unroll
   for (int d = 1; d < kWave; d <<= 1) jmax = max(jmax, __shfl_xor(jmax, d));
   int n = 0;
-  bool ok = true;
+  // loop-carried lane flags are 0/1 integers: in vector registers they need
+  // no exec-mask merges at the joins of the loops
+  int ok = 1;
   // Interval k of the list is cached in registers as [clo, chi], with the
   // bounds of its neighbours (below = hi of k - 1, above = lo of k + 1): a run
   // that meets it without reaching a neighbour is united in registers, and
   // only the other runs go through the list in memory.
   int k = 0;
   double clo = INFINITY, chi = -INFINITY, below = -INFINITY, above = INFINITY;
-  bool dirty = false;
+  int dirty = 0;
   auto unite_slow = [&](double s, double e) {
     if (dirty) {
       olo[k * kWave] = clo;
       ohi[k * kWave] = chi;
-      dirty = false;
+      dirty = 0;
     }
     if (!lane_unite(olo, ohi, ocap, n, k, s, e)) {
-      ok = false;
+      ok = 0;
       return;
     }
     clo = olo[k * kWave];
@@ -1067,7 +1069,7 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
     const bool f = want && fast;
     clo = f ? fmin(clo, s) : clo;
     chi = f ? fmax(chi, e) : chi;
-    dirty = dirty || f;
+    dirty = f ? 1 : dirty;
     const bool slow = want && !fast;
     if (ballot(slow) != 0) {
       if (slow) unite_slow(s, e);
@@ -1089,7 +1091,7 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
     load_block(g.m_hi + o, bk.vb);
     bk.q = *reinterpret_cast<const uint4*>(g.m_dy + o);
   };
-  auto compute_block = [&](const Block& bk, int jb, int m, const double* hrow, bool& run, double& rs, double& re) {
+  auto compute_block = [&](const Block& bk, int jb, int m, const double* hrow, int& run, double& rs, double& re) {
     const T* va = bk.va;
     const T* vb = bk.vb;
     int vdy[B];
@@ -1109,12 +1111,12 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
       unite(valid && run && !ext, rs, re);
       rs = ext ? fmin(rs, s) : (valid ? s : rs);
       re = ext ? fmax(re, e) : (valid ? e : re);
-      run = run || valid;
+      run = valid ? 1 : run;
     }
   };
   // lanes that do not need the block (its range is inside their cached
   // interval) neither load nor use it
-  auto process_block = [&](int c, int jb, int m, const double* hrow, bool& run, double& rs, double& re,
+  auto process_block = [&](int c, int jb, int m, const double* hrow, int& run, double& rs, double& re,
                            bool need) {
     Block bk = {};
     if (need) load_item(c, jb, bk);
@@ -1159,7 +1161,7 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
       for (int j = 0; j < G; ++j) {
         const int ad = (grp * G + j + 1) >> 1;
         const double hmax = ad <= rint ? htab[ad * (rint + 1)] : 0.0;
-        const bool in = pm[j] == 0 || !ok || (double(plo[j]) - hmax >= clo && double(phi[j]) + hmax <= chi);
+        const bool in = pm[j] == 0 || ok == 0 || (double(plo[j]) - hmax >= clo && double(phi[j]) + hmax <= chi);
         need |= ballot(!in) != 0 ? 1u << j : 0u;
       }
       if ((grp + 1) * G < nsteps) fetch(grp + 1);
@@ -1172,7 +1174,7 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
         const int ad = (step + 1) >> 1;
         const double* hrow = htab + ad * (rint + 1);
         const double hmax = hrow[0];
-        bool run = false;
+        int run = 0;
         double rs = 0, re = 0;
         int jb = 0;
         T blo = g.s_lo[summary_index(g.ny, g.caps, c, yc, 0)];
@@ -1184,7 +1186,7 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
             nblo = g.s_lo[summary_index(g.ny, g.caps, c, yc, jb + B)];
             nbhi = g.s_hi[summary_index(g.ny, g.caps, c, yc, jb + B)];
           }
-          const bool inside = jb >= m || !ok || (double(blo) - hmax >= clo && double(bhi) + hmax <= chi);
+          const bool inside = jb >= m || ok == 0 || (double(blo) - hmax >= clo && double(bhi) + hmax <= chi);
           if (ballot(!inside) != 0) process_block(c, jb, m, hrow, run, rs, re, !inside);
           if (!more) break;
           jb += B;
@@ -1206,7 +1208,7 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
         if (c < 0 || c >= g.nx) continue;  // uniform
         const int m = count_of(c);
         if (ballot(m > jb && ok) == 0) continue;
-        bool run = false;
+        int run = 0;
         double rs = 0, re = 0;
         process_block(c, jb, m, htab + ad * (rint + 1), run, rs, re, true);
         unite(run && ok, rs, re);
@@ -1217,7 +1219,7 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
     olo[k * kWave] = clo;
     ohi[k * kWave] = chi;
   }
-  if (ballot(active && !ok) != 0) {
+  if (ballot(active && ok == 0) != 0) {
     if (!g.tile_list)
       raise_status(g.status, kStatusGatherOverflow);  // batched: the host redoes this volume alone
     else if (L == 0)
