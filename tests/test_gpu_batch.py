@@ -37,6 +37,17 @@ def test_batch_equals_single_path_fp64(hip_fp64):
         assert len(bad) == 0 and err <= 1e-9, (op, r, bad[:5], err)
 
 
+def test_batch_large_radii_fp64(hip_fp64):
+    # radii beyond one coverage word per lane (R > 31 and R > 63) in one batch
+    items = mixed_items(3, n=6, side=(8, 30), r=(20.0, 80.0))
+    items = [(op, synth.poisson_columns(vol.nx, vol.ny, lam=4, zmax=300, seed=40 + k, z_margin=100), r)
+             for k, (op, vol, r) in enumerate(items)]
+    res, _, _ = hip_fp64.apply_batch(items)
+    for (op, vol, r), got in zip(items, res):
+        bad, err = compare_csr(got.off, got.z, *oracle.morph(vol, op, r))
+        assert len(bad) == 0 and err <= 1e-9, (op, r, bad[:5], err)
+
+
 def test_batch_fp32_against_oracle(hip_fp32):
     items = mixed_items(2, n=12)
     res, _, _ = hip_fp32.apply_batch(items)

@@ -27,6 +27,12 @@ RANDOM_CASES = [
     (16, 64, 10, 512, 7.5),
     (64, 9, 2, 512, 3.0),
     (50, 50, 6, 1024, 12.25),
+    # larger radii: two coverage words per lane with lanes walking their own
+    # events (R <= 40), two words with every lane stepping through every
+    # event, four words (R >= 64); windows taller than one 64-column tile
+    (12, 150, 5, 512, 33.0),
+    (10, 180, 6, 512, 47.5),
+    (8, 220, 4, 512, 70.0),
 ]
 
 
@@ -101,7 +107,7 @@ EDGE_CASES = {
 
 
 @pytest.mark.parametrize("name", sorted(EDGE_CASES))
-@pytest.mark.parametrize("r", [0.0, 0.5, 1.0, 2.0, 3.7])
+@pytest.mark.parametrize("r", [0.0, 0.5, 1.0, 2.0, 3.7, 33.0, 70.0])
 def test_edge_cases_fp64(hip_fp64, name, r):
     vol = EDGE_CASES[name]()
     res, _, _ = hip_fp64.dilation(vol, r)
