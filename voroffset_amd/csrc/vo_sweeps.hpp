@@ -1582,7 +1582,8 @@ constexpr int kMaxSlabs = 64;    // z-slabs per window
 template <class T, int ECAP, int NB, int WMAX>
 struct Pass1ZSmem {
   T ez[ECAP];
-  int ep[ECAP];  // (window column << 2) | kind: 0 = start, 1 = point, 2 = end
+  uint16_t ep[ECAP];  // (window column << 2) | kind: 0 = start, 1 = point, 2 = end
+  static_assert((WMAX << 2) < 65536, "window columns fit the event code");
   int bin[NB + 1];
   int slab_end[kMaxSlabs + 1];
 };
@@ -1888,20 +1889,20 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
           if (ba >= b_lo && ba < b_hi) {
             const int q = atomicAdd(&sm.bin[ba], 1);
             sm.ez[q] = a;
-            sm.ep[q] = (w << 2) | 1;
+            sm.ep[q] = uint16_t((w << 2) | 1);
           }
           return;
         }
         if (ba >= b_lo && ba < b_hi) {
           const int q = atomicAdd(&sm.bin[ba], 1);
           sm.ez[q] = a;
-          sm.ep[q] = (w << 2) | 0;
+          sm.ep[q] = uint16_t((w << 2) | 0);
         }
         const int bb = bin_of(b);
         if (bb >= b_lo && bb < b_hi) {
           const int q = atomicAdd(&sm.bin[bb], 1);
           sm.ez[q] = b;
-          sm.ep[q] = (w << 2) | 2;
+          sm.ep[q] = uint16_t((w << 2) | 2);
         }
       });
     }
@@ -1920,7 +1921,7 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
           --j;
         }
         sm.ez[j + 1] = zi;
-        sm.ep[j + 1] = pi;
+        sm.ep[j + 1] = uint16_t(pi);
       }
     }
     wave_sync();
