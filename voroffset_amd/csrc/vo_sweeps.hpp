@@ -1590,12 +1590,13 @@ struct Pass1ZSmem {
 
 // Visits the merged segments of one input column (touching or overlapping
 // neighbours fused, points kept only when isolated).
+// Two segments are loaded per step, so that a column costs half as many
+// memory round trips.
 template <class T, class F>
 __device__ __forceinline__ void column_segments(const T* z, int64_t s, int64_t n, F&& f) {
   if (n < 2) return;
   T ca = z[s], cb = z[s + 1];
-  for (int64_t k = s + 2; k + 1 < s + n; k += 2) {
-    const T a = z[k], b = z[k + 1];
+  auto step = [&](T a, T b) {
     if (a <= cb) {
       if (b > cb) cb = b;
     } else {
@@ -1603,7 +1604,14 @@ __device__ __forceinline__ void column_segments(const T* z, int64_t s, int64_t n
       ca = a;
       cb = b;
     }
+  };
+  int64_t k = s + 2;
+  for (; k + 3 < s + n; k += 4) {
+    const T a0 = z[k], b0 = z[k + 1], a1 = z[k + 2], b1 = z[k + 3];
+    step(a0, b0);
+    step(a1, b1);
   }
+  if (k + 1 < s + n) step(z[k], z[k + 1]);
   f(ca, cb);
 }
 
