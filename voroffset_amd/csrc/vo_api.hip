@@ -248,32 +248,38 @@ void launch_zsweep_events(vo_context* ctx, int events, int rint, int64_t grid, c
   }
 }
 
-// Batched z-sweep with the same event capacities.
+// Batched z-sweep with the same event capacities, over the global tiles
+// [t0, t0 + grid) (one radius class of the batch).
 template <class T, int ECAP, int NB>
 void launch_zsweep_batch(vo_context* ctx, int max_rint, int64_t grid, const vo::Pass1ZArgs<T>* jobs,
-                         const int64_t* tile_base, int njobs) {
+                         const int64_t* tile_base, int njobs, int64_t t0) {
   const dim3 g(static_cast<unsigned>(grid)), b(static_cast<unsigned>(vo::kWave));
   if (max_rint <= 31)
     hipLaunchKernelGGL((vo::pass1_zsweep_batch_kernel<T, ECAP, NB, kZWMax, 1>), g, b, 0, ctx->stream, jobs, tile_base,
-                       njobs);
+                       njobs, t0);
   else if (max_rint <= 63)
     hipLaunchKernelGGL((vo::pass1_zsweep_batch_kernel<T, ECAP, NB, kZWMax, 2>), g, b, 0, ctx->stream, jobs, tile_base,
-                       njobs);
+                       njobs, t0);
   else
     hipLaunchKernelGGL((vo::pass1_zsweep_batch_kernel<T, ECAP, NB, kZWMax, 4>), g, b, 0, ctx->stream, jobs, tile_base,
-                       njobs);
+                       njobs, t0);
 }
 
 template <class T>
 void launch_zsweep_batch_events(vo_context* ctx, int events, int max_rint, int64_t grid,
-                                const vo::Pass1ZArgs<T>* jobs, const int64_t* tile_base, int njobs) {
+                                const vo::Pass1ZArgs<T>* jobs, const int64_t* tile_base, int njobs, int64_t t0) {
   switch (events) {
-    case 512: launch_zsweep_batch<T, 512, 128>(ctx, max_rint, grid, jobs, tile_base, njobs); break;
-    case 1024: launch_zsweep_batch<T, 1024, 256>(ctx, max_rint, grid, jobs, tile_base, njobs); break;
-    case 1536: launch_zsweep_batch<T, 1536, 512>(ctx, max_rint, grid, jobs, tile_base, njobs); break;
-    default: launch_zsweep_batch<T, 768, 256>(ctx, max_rint, grid, jobs, tile_base, njobs); break;
+    case 512: launch_zsweep_batch<T, 512, 128>(ctx, max_rint, grid, jobs, tile_base, njobs, t0); break;
+    case 1024: launch_zsweep_batch<T, 1024, 256>(ctx, max_rint, grid, jobs, tile_base, njobs, t0); break;
+    case 1536: launch_zsweep_batch<T, 1536, 512>(ctx, max_rint, grid, jobs, tile_base, njobs, t0); break;
+    default: launch_zsweep_batch<T, 768, 256>(ctx, max_rint, grid, jobs, tile_base, njobs, t0); break;
   }
 }
+
+// Radius class of a batched job: the pass-1 lane state (one coverage word up
+// to R = 31, two up to 63, four up to 127) and, up to R = 31, the H table in
+// LDS for the gather.
+int radius_class(int rint) { return rint <= 31 ? 0 : (rint <= 63 ? 1 : 2); }
 
 int events_for(const vo_context* ctx, int rint) {
   return ctx->zsweep_events ? ctx->zsweep_events : ctx->tuning.events[tune_bucket(rint)];
@@ -561,17 +567,21 @@ void dilate_batch(vo_context* ctx, std::vector<DevVol>& vol, const std::vector<d
     else
       jobs.push_back(j);
   }
+  // jobs grouped by radius class, so that each class is one range of tiles
+  // launched with the kernels for its radii
+  std::stable_sort(jobs.begin(), jobs.end(), [&](int a, int b) {
+    return radius_class(int(std::floor(R[a]))) < radius_class(int(std::floor(R[b])));
+  });
   const int K = int(jobs.size());
   if (K > 0) {
     const int caps = ctx->p1_caps;
     std::vector<int64_t> col_base(K + 1, 0), tile_base(K + 1, 0);
-    int max_rint = 0, ocap = 0;
+    int ocap = 0;
     for (int k = 0; k < K; ++k) {
       const DevVol& v = vol[jobs[k]];
       const int rint = int(std::floor(R[jobs[k]]));
       col_base[k + 1] = col_base[k] + int64_t(v.nx) * v.ny;
       tile_base[k + 1] = tile_base[k] + int64_t(v.nx) * ((v.ny + vo::kGatherTile - 1) / vo::kGatherTile);
-      max_rint = std::max(max_rint, rint);
       ocap = std::max(ocap, ocap_for(ctx, rint));
     }
     std::vector<double> radii(K);
@@ -622,15 +632,35 @@ void dilate_batch(vo_context* ctx, std::vector<DevVol>& vol, const std::vector<d
     // assembly from reading them (those volumes are redone alone)
     check(ctx, hipMemsetAsync(o_len, 0, size_t(C + 1) * sizeof(int64_t), ctx->stream), "memset lengths");
     check(ctx, hipEventRecord(ctx->ev[0], ctx->stream), "event");
-    const dim3 grid(static_cast<unsigned>(NT)), block(static_cast<unsigned>(vo::kWave));
-    launch_zsweep_batch_events<T>(ctx, events_for(ctx, max_rint), max_rint, NT, d1, dt, K);
+    // tile range and largest radius of each class
+    int64_t cls_t0[3] = {0, 0, 0}, cls_t1[3] = {0, 0, 0};
+    int cls_rmax[3] = {0, 0, 0};
+    for (int k = K - 1; k >= 0; --k) {
+      const int rint = int(std::floor(R[jobs[k]]));
+      const int c = radius_class(rint);
+      if (cls_t1[c] == 0) cls_t1[c] = tile_base[k + 1];
+      cls_t0[c] = tile_base[k];
+      cls_rmax[c] = std::max(cls_rmax[c], rint);
+    }
+    const dim3 block(static_cast<unsigned>(vo::kWave));
+    for (int c = 0; c < 3; ++c) {
+      if (cls_t1[c] > cls_t0[c])
+        launch_zsweep_batch_events<T>(ctx, events_for(ctx, cls_rmax[c]), cls_rmax[c], cls_t1[c] - cls_t0[c], d1, dt,
+                                      K, cls_t0[c]);
+    }
     check(ctx, hipGetLastError(), "batched pass1 launch");
     check(ctx, hipEventRecord(ctx->ev[1], ctx->stream), "event");
-    if (max_rint <= vo::kGatherLdsTableMax)
-      hipLaunchKernelGGL((vo::pass2_gather_batch_kernel<T, true>), grid, block,
-                         size_t(max_rint + 1) * (max_rint + 1) * sizeof(double), ctx->stream, d2, dt, K);
-    else
-      hipLaunchKernelGGL((vo::pass2_gather_batch_kernel<T, false>), grid, block, 0, ctx->stream, d2, dt, K);
+    for (int c = 0; c < 3; ++c) {
+      if (cls_t1[c] <= cls_t0[c]) continue;
+      const dim3 grid(static_cast<unsigned>(cls_t1[c] - cls_t0[c]));
+      if (cls_rmax[c] <= vo::kGatherLdsTableMax)
+        hipLaunchKernelGGL((vo::pass2_gather_batch_kernel<T, true>), grid, block,
+                           size_t(cls_rmax[c] + 1) * (cls_rmax[c] + 1) * sizeof(double), ctx->stream, d2, dt, K,
+                           cls_t0[c]);
+      else
+        hipLaunchKernelGGL((vo::pass2_gather_batch_kernel<T, false>), grid, block, 0, ctx->stream, d2, dt, K,
+                           cls_t0[c]);
+    }
     check(ctx, hipGetLastError(), "batched pass2 launch");
     check(ctx, hipEventRecord(ctx->ev[2], ctx->stream), "event");
     // output CSR over all columns of the batch

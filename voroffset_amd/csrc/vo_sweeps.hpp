@@ -1253,8 +1253,9 @@ __global__ __launch_bounds__(kWave, GatherWaves<T>::value) void pass2_gather_ker
 // Batched: job j owns the global tiles [tile_base[j], tile_base[j + 1]).
 template <class T, bool kLdsTable>
 __global__ __launch_bounds__(kWave, GatherWaves<T>::value) void pass2_gather_batch_kernel(const Pass2GArgs<T>* jobs,
-                                                                   const int64_t* tile_base, int njobs) {
-  const int64_t t = xcd_tile(blockIdx.x, gridDim.x);
+                                                                   const int64_t* tile_base, int njobs,
+                                                                   int64_t t0) {
+  const int64_t t = t0 + xcd_tile(blockIdx.x, gridDim.x);
   const int j = find_job(tile_base, njobs, t);
   const Pass2GArgs<T> g = jobs[j];
   gather_tile<T, kLdsTable>(g, t - tile_base[j]);
@@ -1994,9 +1995,10 @@ __global__ __launch_bounds__(kWave) void pass1_zsweep_kernel(Pass1ZArgs<T> g) {
 // Batched: job j owns the global tiles [tile_base[j], tile_base[j + 1]).
 template <class T, int ECAP, int NB, int WMAX, int NW>
 __global__ __launch_bounds__(kWave) void pass1_zsweep_batch_kernel(const Pass1ZArgs<T>* jobs,
-                                                                   const int64_t* tile_base, int njobs) {
+                                                                   const int64_t* tile_base, int njobs,
+                                                                   int64_t t0) {
   __shared__ Pass1ZSmem<T, ECAP, NB, WMAX> sm;
-  const int64_t t = xcd_tile(blockIdx.x, gridDim.x);
+  const int64_t t = t0 + xcd_tile(blockIdx.x, gridDim.x);
   const int j = find_job(tile_base, njobs, t);
   const Pass1ZArgs<T> g = jobs[j];
   zsweep_tile<T, ECAP, NB, WMAX, NW>(g, t - tile_base[j], sm);
