@@ -1063,6 +1063,21 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
   };
   // unites [s, e] where `want` holds; written with selects so that the
   // wavefront branches only when some lane needs the list
+  // A run that needs the list is deferred: each lane holds at most one such
+  // run, and the held runs of all lanes go through the list together at the
+  // end of the block (or when a lane needs a second one), so that lanes
+  // needing the list at different triples share one pass of the wavefront
+  // through that path.  The union does not depend on the order of its
+  // pieces, and a block is skipped only when it lies inside the cached
+  // interval, which is part of the union whatever is still held.
+  double hs = 0, he = 0;
+  int held = 0;
+  auto flush_held = [&]() {
+    if (ballot(held != 0) != 0) {
+      if (held != 0) unite_slow(hs, he);
+      held = 0;
+    }
+  };
   auto unite = [&](bool want, double s, double e) {
     if (ballot(want) == 0) return;  // the usual case inside a run
     const bool fast = s <= chi && e >= clo && s > below && e < above;
@@ -1071,9 +1086,13 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
     chi = f ? fmax(chi, e) : chi;
     dirty = f ? 1 : dirty;
     const bool slow = want && !fast;
-    if (ballot(slow) != 0) {
-      if (slow) unite_slow(s, e);
+    if (ballot(slow && held != 0) != 0) {
+      if (slow && held != 0) unite_slow(hs, he);
+      held = (slow && held != 0) ? 0 : held;
     }
+    hs = slow ? s : hs;
+    he = slow ? e : he;
+    held = slow ? 1 : held;
   };
   constexpr int B = kSlotBlock;
   // The lane's block of B contiguous triples of column c from jb on (jb is a
@@ -1113,6 +1132,7 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
       re = ext ? fmax(re, e) : (valid ? e : re);
       run = valid ? 1 : run;
     }
+    flush_held();
   };
   // lanes that do not need the block (its range is inside their cached
   // interval) neither load nor use it
@@ -1215,6 +1235,7 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
       }
     }
   }
+  flush_held();
   if (dirty && ok) {
     olo[k * kWave] = clo;
     ohi[k * kWave] = chi;
