@@ -1065,9 +1065,9 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
   // wavefront branches only when some lane needs the list
   // A run that needs the list is deferred: each lane holds at most one such
   // run, and the held runs of all lanes go through the list together at the
-  // end of the block (or when a lane needs a second one), so that lanes
-  // needing the list at different triples share one pass of the wavefront
-  // through that path.  The union does not depend on the order of its
+  // end of the source column (or when a lane needs a second one), so that
+  // lanes needing the list at different triples share one pass of the
+  // wavefront through that path.  The union does not depend on the order of its
   // pieces, and a block is skipped only when it lies inside the cached
   // interval, which is part of the union whatever is still held.
   double hs = 0, he = 0;
@@ -1132,7 +1132,6 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
       re = ext ? fmax(re, e) : (valid ? e : re);
       run = valid ? 1 : run;
     }
-    flush_held();
   };
   // lanes that do not need the block (its range is inside their cached
   // interval) neither load nor use it
@@ -1214,6 +1213,7 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
           bhi = nbhi;
         }
         unite(run && ok, rs, re);
+        flush_held();
       }
     }
   } else {
@@ -1232,6 +1232,7 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
         double rs = 0, re = 0;
         process_block(c, jb, m, htab + ad * (rint + 1), run, rs, re, true);
         unite(run && ok, rs, re);
+        flush_held();
       }
     }
   }
