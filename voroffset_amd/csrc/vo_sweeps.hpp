@@ -902,12 +902,13 @@ __global__ void finalize_offsets_kernel(const FinalJob* jobs, const int64_t* ent
 // H = sqrt(R^2 - dy^2 - d^2): exactly what SeparatePower2D's forward and
 // backward sweeps assemble (their getLine emits [y1 - dz, y2 + dz] for every
 // live interior and point, SeparatePower2D.cpp:295-341).  One wavefront per
-// tile of 64 output columns (x, y0 .. y0 + 63), one lane per column.  Piece j
-// of source column (x + d, y) is read with one coalesced load per d for the
-// whole wavefront (the slot layout is y-minor), dilated with H from an LDS
-// table, and united into the lane's sorted output list in LDS, whose current
-// interval is cached in registers.  A tile whose lists overflow is recomputed
-// by the sweep form, column by column.
+// tile of 64 output columns (x, y0 .. y0 + 63), one lane per column.  A block
+// of 8 pieces of source column (x + d, y) is read with 16-byte loads, 2 KB
+// contiguous per wavefront (the slot layout keeps the blocks of consecutive
+// y together), dilated with H from a table (in LDS for R <= 31), and united
+// into the lane's sorted output list, kept in global memory with its current
+// interval cached in registers.  A tile whose lists overflow makes the
+// gather run again with more room, then falls back to the sweep form.
 // ---------------------------------------------------------------------------
 
 namespace vo {
