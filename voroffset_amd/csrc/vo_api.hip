@@ -319,6 +319,7 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
   int chunk1 = std::min(Shapes<T>::p1_cmax, 32);
   int chunk2 = Shapes<T>::p2_cmax;
   bool zsweep_failed = false;  // a window too dense for the z-sweep form: sweep along y instead
+  int64_t fast_total = -1;     // the output size, when the gather's one synchronisation settled it
   for (int attempt = 0;; ++attempt) {
     const int nch1 = (ny + chunk1 - 1) / chunk1;
     const int64_t grid1 = int64_t(nx) * nch1;
@@ -387,7 +388,22 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
           hipLaunchKernelGGL((vo::pass2_gather_kernel<T, false>), dim3(unsigned(ntiles)), dim3(vo::kWave), 0,
                              ctx->stream, ag);
         check(ctx, hipGetLastError(), "pass2 gather launch");
-        nfall = int(read_word32(ctx, tiles));
+        // The output scan is launched before the one synchronisation that
+        // reads the overflow count and the status: when neither asks for
+        // more work, its total is final and the dilation is done.
+        check(ctx, hipEventRecord(ctx->ev[2], ctx->stream), "event");
+        check(ctx, hipMemsetAsync(o_len + ncols, 0, sizeof(int64_t), ctx->stream), "memset");
+        exclusive_scan(ctx, o_len, out_off, ncols + 1);
+        check(ctx, hipMemcpyAsync(ctx->host_words + 2, tiles, sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
+              "copy word");
+        check(ctx, hipMemcpyAsync(ctx->host_words + 1, status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
+              "copy status");
+        check(ctx, hipMemcpyAsync(ctx->host_words, out_off + ncols, sizeof(int64_t), hipMemcpyDeviceToHost,
+                                  ctx->stream),
+              "copy word");
+        check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+        std::memcpy(&nfall, ctx->host_words + 2, sizeof(int));
+        if (nfall == 0 && int(ctx->host_words[1] & 0xffffffff) == 0) fast_total = ctx->host_words[0];
         if (nfall == 0 || gocap >= 64) break;
         // lists outgrew their capacity: the whole gather again with twice the
         // room is cheaper and more robust than the sweep form for many tiles
@@ -407,6 +423,16 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
       check(ctx, hipGetLastError(), "pass2 launch");
     }
     ctx->last_fallback_tiles = nfall;
+    if (fast_total >= 0) {
+      if (t1 || t2) {
+        float ms1 = 0, ms2 = 0;
+        check(ctx, hipEventElapsedTime(&ms1, ctx->ev[0], ctx->ev[1]), "elapsed");
+        check(ctx, hipEventElapsedTime(&ms2, ctx->ev[1], ctx->ev[2]), "elapsed");
+        if (t1) *t1 = ms1;
+        if (t2) *t2 = ms2;
+      }
+      break;
+    }
     check(ctx, hipEventRecord(ctx->ev[2], ctx->stream), "event");
     check(ctx, hipMemsetAsync(o_len + ncols, 0, sizeof(int64_t), ctx->stream), "memset");
     check(ctx, hipMemcpyAsync(ctx->host_words + 1, status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
@@ -437,9 +463,12 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
     ctx->p1_caps *= 2;
     ctx->p2_cap_per_col *= 2;
   }
-  // output CSR
-  exclusive_scan(ctx, o_len, out_off, ncols + 1);
-  const int64_t total = read_word(ctx, out_off + ncols);
+  // output CSR (scanned already when the gather settled it)
+  int64_t total = fast_total;
+  if (total < 0) {
+    exclusive_scan(ctx, o_len, out_off, ncols + 1);
+    total = read_word(ctx, out_off + ncols);
+  }
   T* oz = static_cast<T*>(ws(ctx, out_z_slot, size_t(std::max<int64_t>(total, 1)) * sizeof(T)));
   hipLaunchKernelGGL(vo::gather_kernel<T>, dim3(blocks_for(ncols, 256)), dim3(256), 0, ctx->stream, ncols,
                      static_cast<const T*>(ctx->buf[kO_Z].ptr), o_start, o_len, out_off, oz);
