@@ -45,6 +45,17 @@ def _crop(vol, crop):
                                          spacing=vol.spacing, padding=vol.padding)
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(vol, radius, threads, crop, crop1):
     """The oracle's `ours` (FP64 restatement, line-parallel with grain 10
     lines like the reference's dormant TBB path) on crop x crop of the
@@ -70,7 +81,122 @@ def cpu_baseline(vol, radius, threads, crop, crop1):
                          f"{threads} threads, {dt:.1f} s",
                "single_thread": {"value": round(single.ncols / dt1, 1), "cores": 1,
                                  "sample": f"{crop1}x{crop1} crop, 1 thread, {dt1:.1f} s"},
-               "host_cpus": os.cpu_count()}
+               "host_cpus": os.cpu_count(), "cpu_model": cpu_model(),
+               "oracle": "the repository's FP64 CPU restatement of `ours` (oracle/vo_oracle.cpp), pinned by "
+                         "closed forms, `ours` == `brute_force` on all four operations and the pass-1 envelope "
+                         "identity; the reference itself cannot be compiled here (its core needs Eigen)"}
+
+
+# -- precision report ----------------------------------------------------------
+# A check item is one GPU result (FP32 storage) on a window, beside the FP64
+# input it was computed from, cropped with enough margin that the window's
+# columns see all the input they depend on (the radius, twice for closing and
+# opening; a crop that reaches the grid border keeps that border).  The
+# cpu_baseline leg runs the oracle on each crop and compares the window.
+
+def _reach(op, r):
+    k = int(math.floor(r))
+    return 2 * k + 1 if op in ("closing", "opening") else k + 1
+
+
+def window_item(label, vol, res, op, r, x0, y0, w):
+    """A check item for the w x w window at (x0, y0) of `res` = op(vol)."""
+    m = _reach(op, r)
+    cx0, cy0 = max(0, x0 - m), max(0, y0 - m)
+    cx1, cy1 = min(vol.nx, x0 + w + m), min(vol.ny, y0 + w + m)
+    win = res.crop(x0, y0, w, w)
+    return {"config": label, "op": op, "r": r, "input": vol.crop(cx0, cy0, cx1 - cx0, cy1 - cy0),
+            "window": (x0 - cx0, y0 - cy0, w, w), "off": win.off, "z": win.z}
+
+
+def whole_item(label, vol, res, op, r):
+    return {"config": label, "op": op, "r": r, "input": vol, "window": (0, 0, vol.nx, vol.ny),
+            "off": res.off, "z": res.z}
+
+
+def direct_items(vol, windows=((500, 300), (131, 877)), w=24):
+    """Config 3 (the bench volume): dilation and erosion at r = 16 and 64,
+    closing at r = 16, FP32 storage through the host entry point."""
+    from voroffset_amd.morpho import VoronoiMorphoHip
+
+    hip = VoronoiMorphoHip("fp32")
+    items = []
+    try:
+        for op, r in (("dilation", 16.0), ("dilation", 64.0), ("erosion", 16.0), ("erosion", 64.0),
+                      ("closing", 16.0)):
+            res, _, _ = hip.apply(op, vol, r)
+            items += [window_item(f"direct {vol.nx}^2", vol, res, op, r, x, y, w) for x, y in windows]
+    finally:
+        hip.close()
+    return items
+
+
+def mesh_items():
+    """Configs 1 and 2: the 32-sphere mesh (128 dexels, padding 8, dilation
+    r = 4, whole volume) and the 2000-tube mesh (512 box, padding 16:
+    dilation 10, erosion 3, opening / closing 5, on two windows), rasterized
+    and processed on the GPU."""
+    from voroffset_amd import mesh as M
+    from voroffset_amd.morpho import VoronoiMorphoHip
+
+    hip = VoronoiMorphoHip("fp32")
+    items = []
+    try:
+        spheres = M.create_dexels(M.icosphere_union(32, level=3, seed=1), 1.0, 8, 128, ctx=hip.ctx)
+        res, _, _ = hip.apply("dilation", spheres, 4.0)
+        items.append(whole_item("spheres 128", spheres, res, "dilation", 4.0))
+        tubes = M.create_dexels(M.prism_tubes(2000, box=512.0, seed=2), 1.0, 16, -1, ctx=hip.ctx)
+        for op, r in (("dilation", 10.0), ("erosion", 3.0), ("opening", 5.0), ("closing", 5.0)):
+            res, _, _ = hip.apply(op, tubes, r)
+            items += [window_item("tubes 512", tubes, res, op, r, x, y, 32)
+                      for x, y in ((tubes.nx // 2, tubes.ny // 3), (100, 380))]
+    finally:
+        hip.close()
+    return items
+
+
+def precision_report(items, threads):
+    """Per config: the max endpoint error of the FP32-storage GPU results
+    against the FP64 oracle on the FP64 input (so it includes the float32
+    rounding of the input), columns compared, columns whose segment counts
+    differ, and how many of those only differ by gaps narrower than 1e-5
+    dexel (the rest are listed as unexplained)."""
+    import oracle
+
+    rep = {}
+    for it in items:
+        ro, rz = oracle.morph(it["input"], it["op"], it["r"], threads=threads)
+        ref = it["input"].like(ro, rz).crop(*it["window"])
+        la, lb = np.diff(it["off"]), np.diff(ref.off)
+        bad = np.nonzero(la != lb)[0]
+        same = np.nonzero((la == lb) & (la > 0))[0]
+        err = 0.0
+        if same.size:
+            ia = np.concatenate([np.arange(it["off"][c], it["off"][c + 1]) for c in same])
+            ib = np.concatenate([np.arange(ref.off[c], ref.off[c + 1]) for c in same])
+            err = float(np.max(np.abs(it["z"][ia] - ref.z[ib])))
+
+        def fused(z):
+            out = []
+            for k in range(0, len(z), 2):
+                if out and z[k] - out[-1] < 1e-5:
+                    out[-1] = max(out[-1], z[k + 1])
+                else:
+                    out.extend([z[k], z[k + 1]])
+            return len(out)
+        narrow = sum(1 for c in bad if fused(it["z"][it["off"][c]:it["off"][c + 1]]) ==
+                     fused(ref.z[ref.off[c]:ref.off[c + 1]]))
+        key = f'{it["config"]}: {it["op"]} r={it["r"]:g}'
+        e = rep.setdefault(key, {"max_err": 0.0, "columns": 0, "count_mismatch": 0, "narrow_gap": 0,
+                                 "unexplained": 0})
+        e["max_err"] = max(e["max_err"], err)
+        e["columns"] += int(la.size)
+        e["count_mismatch"] += int(bad.size)
+        e["narrow_gap"] += int(narrow)
+        e["unexplained"] += int(bad.size - narrow)
+    for e in rep.values():
+        e["max_err"] = float(f'{e["max_err"]:.3g}')
+    return rep
 
 
 def _bucket(radius):
@@ -164,6 +290,21 @@ def batched_leg(n_volumes, steps, warmup, seed=4321, tune=True):
         plain()
     _native.synchronize()
     tp = (time.perf_counter() - t) / steps
+    # results of four sampled volumes of side <= 256 for the precision report
+    from voroffset_amd.volume import CompressedVolume
+    outs, _ = ctx.morph_batch_device("fp32", jobs32)
+    rng = np.random.default_rng(seed)
+    small = [k for k, sp in enumerate(specs) if sp["side"] <= 256]
+    items = []
+    for k in sorted(rng.choice(small, min(4, len(small)), replace=False).tolist()):
+        v, cnt = outs[k]
+        sp = specs[k]
+        src = synth.poisson_columns(sp["side"], sp["side"], lam=sp["lam"], zmax=1024, seed=sp["seed"])
+        zlo, zhi = src.z_range()
+        shift = float(np.round(0.5 * (zlo + zhi)))
+        res = src.like(_native.to_host(v.offsets, np.int64, v.nx * v.ny + 1),
+                       _native.to_host(v.z, np.float32, cnt).astype(np.float64) + shift)
+        items.append(whole_item(f"batch of {n_volumes}", src, res, sp["op"], sp["radius"]))
     ctx.close()
     for d in keep:
         d.free()
@@ -173,7 +314,7 @@ def batched_leg(n_volumes, steps, warmup, seed=4321, tune=True):
                         f"r U[2,32], ops uniform over dilation/erosion/closing/opening, seed {seed}",
             "columns": cols, "batched_fp32": {"value": round(cols / tb, 1), "ms_per_batch": round(tb * 1e3, 3)},
             "plain_fp64": {"value": round(cols / tp, 1), "ms_per_batch": round(tp * 1e3, 3)},
-            "batched_over_plain": round(tp / tb, 3), "volumes_redone_alone": alone}
+            "batched_over_plain": round(tp / tb, 3), "volumes_redone_alone": alone}, items
 
 
 def served_leg(rate, n_requests, pool, seed=99, tune=True):
@@ -206,6 +347,19 @@ def served_leg(rate, n_requests, pool, seed=99, tune=True):
         span = max(r.done for r in done) - min(r.arrival for r in done)
         rep = serve.latency_report(done)
         total = sum(cols[r.handle] for r in done)
+        # two results on the largest resident volume for the precision report
+        big = int(np.argmax(sides))
+        src = synth.poisson_columns(int(sides[big]), int(sides[big]), lam=8, zmax=1024, seed=seed * 1000 + big)
+        items = []
+        for op, r in (("closing", 48.0), ("dilation", 30.5)):
+            req = srv.submit(handles[big], op, r)
+            while srv.queue:
+                srv.step()
+            res = srv.get(req.result, src)
+            srv.drop(req.result)
+            s0 = src.nx
+            items += [window_item(f"served {s0}^2", src, res, op, r, x, y, 24) for x, y in ((s0 // 2, s0 // 3),
+                                                                                         (s0 - 24, 0))]
         rep.update({"workload": f"{pool} resident volumes, side log-uniform [128, 2048], Poisson(8) segments/column; "
                                 f"{n_requests} requests, Poisson arrivals at {rate:g}/s, op uniform, r U[2,48]",
                     "offered_rate_per_s": rate, "columns_per_s": round(total / span, 1),
@@ -214,7 +368,7 @@ def served_leg(rate, n_requests, pool, seed=99, tune=True):
                     "wall_s": round(time.perf_counter() - t, 3)})
         log(f"served: p50 {rep['p50_ms']} ms, p99 {rep['p99_ms']} ms, {rep['requests_per_s']} req/s, "
             f"{rep['columns_per_s']:.3g} columns/s, {rep['batches']} batches")
-        return rep
+        return rep, items
     finally:
         srv.close()
 
@@ -232,6 +386,7 @@ def main():
     ap.add_argument("--cpu-crop1", type=int, default=192, help="crop side of the single-thread CPU sample")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 leg (counter runs)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default shapes (skip the autotuner)")
+    ap.add_argument("--no-precision", action="store_true", help="skip the precision report (oracle checks)")
     ap.add_argument("--batch-volumes", type=int, default=256, help="volumes of the batched leg (0: skip it)")
     ap.add_argument("--serve-requests", type=int, default=200, help="requests of the served leg (0: skip it)")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="offered requests per second")
@@ -273,15 +428,27 @@ def main():
 
     # the mixed legs keep the default shapes: tuned on one sample they ran
     # slower on the uneven volumes (measured: batch 110 ms tuned, 89 ms not)
-    batched = batched_leg(args.batch_volumes, max(2, args.steps // 5), 1, tune=False) if args.batch_volumes > 0 else None
+    items = []
+    batched = None
+    if args.batch_volumes > 0:
+        batched, more = batched_leg(args.batch_volumes, max(2, args.steps // 5), 1, tune=False)
+        items += more
 
-    served = (served_leg(args.serve_rate, args.serve_requests, args.serve_pool, tune=False)
-              if args.serve_requests > 0 else None)
+    served = None
+    if args.serve_requests > 0:
+        served, more = served_leg(args.serve_rate, args.serve_requests, args.serve_pool, tune=False)
+        items += more
 
-    m, cpu = (None, None)
+    m, cpu, precision = (None, None, None)
     if not args.no_cpu:
         m, cpu = cpu_baseline(vol, R, args.cpu_threads, args.cpu_crop, args.cpu_crop1)
         log(f"cpu baseline: {cpu['value']:.3g} columns/s")
+        if not args.no_precision:  # the oracle as the checker of GPU results computed above
+            t = time.perf_counter()
+            precision = precision_report(direct_items(vol) + mesh_items() + items, args.cpu_threads)
+            log(f"precision report ({time.perf_counter() - t:.1f} s): "
+                f"max err {max(e['max_err'] for e in precision.values()):.3g}, "
+                f"unexplained count mismatches {sum(e['unexplained'] for e in precision.values())}")
 
     E = 4
     s_in = vol.num_segments() / ncols
@@ -320,7 +487,8 @@ def main():
                    "pass1_ms": round(p1_ms, 4), "pass2_ms": round(p2_ms, 4),
                    "plain_fp64": plain, "pass2_sweep_tiles": fallback, "autotuned": tuned, "batched": batched,
                    "served": served,
-                   "s_in": round(s_in, 4), "s_out": round(s_out, 4), "m_oracle": m},
+                   "s_in": round(s_in, 4), "s_out": round(s_out, 4), "m_oracle": m,
+                   "precision": precision},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

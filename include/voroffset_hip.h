@@ -33,6 +33,7 @@ extern "C" {
 typedef struct vo_context vo_context;
 typedef struct vo_result vo_result;
 typedef struct vo_mesh_result vo_mesh_result;
+typedef struct vo_triples vo_triples;
 
 enum vo_status {
   VO_OK = 0,
@@ -55,9 +56,15 @@ typedef struct {
   double z_lo, z_hi;       /* z-range used by erosion (see above) */
 } vo_volume;
 
-/* Context bound to HIP device `device`: owns a stream and a workspace pool. */
+/* Context bound to HIP device `device`: owns a stream and a workspace pool.
+ * Environment read here: VO_INJECT_HIP_ERROR=<label> makes every HIP call of
+ * the context whose internal label contains <label> ("pass1 launch", "sync",
+ * "upload", ...) report VO_ERR_DEVICE after it ran -- a fault hook for the
+ * error paths of callers (the reference's vor_assert failures). */
 int vo_create(vo_context** ctx, int device);
 void vo_destroy(vo_context* ctx);
+/* Text of the last error on `ctx`; with ctx = NULL, why this thread's last
+ * vo_create failed ("no context" when it did not). */
 const char* vo_last_error(const vo_context* ctx);
 
 /* Host volume in (z as double), host result out.  With VO_FP32 the endpoints
@@ -70,9 +77,30 @@ int vo_result_shape(const vo_result* r, int32_t* nx, int32_t* ny, int64_t* count
 int vo_result_copy(const vo_result* r, int64_t* offsets, double* z);
 void vo_result_free(vo_result* r);
 
+/* The pass-1 intermediate of a dilation of a host volume by `radius`: replaces
+ * reading the vor3d::CompressedVolumeWithRadii that VoronoiMorphoVorPower
+ * builds between its passes (src/vor3d/VoronoiVorPower.cpp:39-66,
+ * CompressedVolumeWithRadii.h:10-38, Common.h:119-155).  Per column (index
+ * x + nx*y) the z-sorted, z-disjoint triples (y1, y2, r): for every z in
+ * [y1, y2] the nearest line y' = y +- dy (|dy| <= radius) of the same x whose
+ * input covers z is at distance dy, and r = sqrt(radius^2 - dy^2).  The GPU
+ * keeps every such piece; the reference also drops pieces its sweep finds
+ * dominated, so counts differ while the envelope max_t (r^2 - dist_z(z, t)^2)
+ * is the same.  A library-owned host result (shape / copy / free). */
+int vo_pass1(vo_context* ctx, int precision, double radius, const vo_volume* in, vo_triples** out);
+int vo_triples_shape(const vo_triples* t, int32_t* nx, int32_t* ny, int64_t* count);
+int vo_triples_copy(const vo_triples* t, int64_t* offsets, double* y1, double* y2, double* r);
+void vo_triples_free(vo_triples* t);
+
 /* Device volume in (HBM pointers, z of the given precision), device result
  * out: `out` receives pointers into the context's workspace, valid until the
- * next call on `ctx`.  `*out_count` is the number of endpoints. */
+ * next call on `ctx`.  `*out_count` is the number of endpoints.  The input is
+ * checked on the device as vo_morph checks it on the host: decreasing
+ * offsets, an odd column, unsorted or non-finite endpoints -> VO_ERR_ARGUMENT;
+ * for erosion / opening an endpoint outside [z_lo - 1, z_hi + 1], for a
+ * closing one whose dilation by `radius` leaves it -> VO_ERR_RANGE (the
+ * reference's preconditions, Voronoi.cpp:27-42, Common.h:228).  The offsets
+ * must index the z buffer. */
 int vo_morph_device(vo_context* ctx, int op, int precision, double radius, const vo_volume* in,
                     vo_volume* out, int64_t* out_count, double* time_1_ms, double* time_2_ms);
 
@@ -81,6 +109,8 @@ int vo_morph_device(vo_context* ctx, int op, int precision, double radius, const
  * Voronoi.h:18-29, offset3d.cpp:116-136, over many volumes).  Each job names
  * an op, a radius and a device volume whose z has the batch's precision; all
  * jobs run together, one launch per pass and stage for the whole batch.
+ * Inputs are checked on the device as for vo_morph_device; the error text
+ * names the first failing job.
  * out[i] receives device pointers into the context's workspace (offsets
  * starting at 0), valid until the next call on `ctx`; out_count[i] is its
  * number of endpoints.  time_1 / time_2 sum the batched pass-1 / pass-2
@@ -94,6 +124,13 @@ typedef struct {
 
 int vo_morph_batch_device(vo_context* ctx, int precision, int n, const vo_job* jobs, vo_volume* out,
                           int64_t* out_count, double* time_1_ms, double* time_2_ms);
+
+/* n device-to-device copies dst[i] <- src[i] (bytes[i] each) on the
+ * context's stream, then one synchronisation: how a caller keeps results of
+ * vo_morph_batch_device / vo_morph_device beyond the next call, into buffers
+ * it owns.  No reference counterpart (the reference returns results by
+ * value, Voronoi.h:18). */
+int vo_copy_device(vo_context* ctx, int n, void* const* dst, const void* const* src, const int64_t* bytes);
 
 /* Number of jobs of the last batch on `ctx` that were redone one by one
  * (a tuning signal).  No reference counterpart. */

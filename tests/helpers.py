@@ -39,3 +39,95 @@ def quantize_fp32(vol):
     shift = float(np.round(0.5 * (zlo + zhi)))
     zq = (vol.z - shift).astype(np.float32).astype(np.float64) + shift
     return vol.like(vol.off.copy(), zq)
+
+
+def crop(vol, x0, y0, w, h):
+    """The w x h sub-grid of `vol` starting at (x0, y0), with vol's geometry
+    (the z-range, which erosion depends on, is unchanged)."""
+    from voroffset_amd.volume import CompressedVolume
+    xs = np.arange(x0, x0 + w)
+    cols = (xs[None, :] + vol.nx * np.arange(y0, y0 + h)[:, None]).ravel()
+    lens = vol.off[cols + 1] - vol.off[cols]
+    off = np.zeros(cols.size + 1, np.int64)
+    np.cumsum(lens, out=off[1:])
+    idx = np.repeat(vol.off[cols] - off[:-1], lens) + np.arange(off[-1])
+    return CompressedVolume(w, h, off, vol.z[idx].copy(), vol.origin, vol.extent, vol.spacing, vol.padding,
+                            dict(vol.meta))
+
+
+def csr_valid(off, z, zero_gaps=False):
+    """Every column even-length, each segment non-inverted, consecutive
+    segments of a column separated by a positive gap (touching ones fused).
+    zero_gaps=True also accepts gaps of width exactly 0: float32 storage
+    rounds a gap narrower than the rounding step to that, and keeps the two
+    segments (as the FP64 result has them)."""
+    lens = np.diff(off)
+    if np.any(lens % 2) or off[-1] != z.size:
+        return False
+    if z.size == 0:
+        return True
+    a, b = z[0::2], z[1::2]
+    if np.any(b < a) or not np.all(np.isfinite(z)):
+        return False
+    seg_col = np.repeat(np.arange(lens.size), lens // 2)
+    same = seg_col[1:] == seg_col[:-1]
+    gap = a[1:][same] - b[:-1][same]
+    return bool(np.all(gap >= 0) if zero_gaps else np.all(gap > 0))
+
+
+def zero_gaps(off, z):
+    """Number of gaps of width exactly 0 between consecutive segments of a column."""
+    lens = np.diff(off)
+    if z.size == 0:
+        return 0
+    seg_col = np.repeat(np.arange(lens.size), lens // 2)
+    same = seg_col[1:] == seg_col[:-1]
+    return int(np.count_nonzero(z[2::2][same] == z[1:-1:2][same]))
+
+
+def inside(inner_off, inner_z, outer_off, outer_z, tol=0.0):
+    """(number of segments of `inner` not inside one segment of `outer` in the
+    same column, after shrinking each inner segment by `tol` at both ends).
+    Exact comparisons: a merge of the two sorted CSR lists, no key arithmetic."""
+    n = inner_off.size - 1
+    ci = np.repeat(np.arange(n), np.diff(inner_off) // 2)
+    co = np.repeat(np.arange(n), np.diff(outer_off) // 2)
+    a, b = inner_z[0::2] + tol, inner_z[1::2] - tol
+    keep = a <= b
+    a, b, ci = a[keep], b[keep], ci[keep]
+    if a.size == 0:
+        return 0
+    s, e = outer_z[0::2], outer_z[1::2]
+    col = np.concatenate([co, ci])
+    zz = np.concatenate([s, a])
+    kind = np.concatenate([np.zeros(s.size, np.int8), np.ones(a.size, np.int8)])
+    order = np.lexsort((kind, zz, col))
+    # outer segments are in (column, z) order, so the index of the last outer
+    # start seen is a running maximum
+    tag = np.where(order < s.size, order, -1)
+    last = np.maximum.accumulate(tag)
+    at_inner = order >= s.size
+    j = last[at_inner]
+    ii = order[at_inner] - s.size
+    jj = np.maximum(j, 0)
+    ok = (j >= 0) & (co[jj] == ci[ii]) & (e[jj] >= b[ii]) if s.size else np.zeros(ii.size, bool)
+    return int(np.count_nonzero(~ok))
+
+
+def window_columns(off, z, nx, x0, y0, w, h):
+    """(off, z) of the w x h window at (x0, y0) of a CSR volume with nx columns per row."""
+    xs = np.arange(x0, x0 + w)
+    cols = (xs[None, :] + nx * np.arange(y0, y0 + h)[:, None]).ravel()
+    lens = off[cols + 1] - off[cols]
+    o = np.zeros(cols.size + 1, np.int64)
+    np.cumsum(lens, out=o[1:])
+    idx = np.repeat(off[cols] - o[:-1], lens) + np.arange(o[-1])
+    return o, z[idx]
+
+
+def reach(op, r):
+    """Columns from the crop edge beyond which a cropped input gives the full
+    volume's result: the operation reads input within r of a column (twice
+    for closing and opening); the ring of an erosion sits one column outside."""
+    k = int(np.floor(r))
+    return 2 * k + 1 if op in ("closing", "opening") else k + 1

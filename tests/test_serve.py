@@ -74,3 +74,27 @@ def test_served_results_equal_single_path(prec, hip_fp32, hip_fp64):
         assert all(req.done >= req.start >= req.arrival for req in reqs)
     finally:
         srv.close()
+
+
+def test_pool_reuses_buffers(monkeypatch):
+    made = []
+
+    class FakeBuffer:
+        def __init__(self, nbytes):
+            self.nbytes = nbytes
+            made.append(self)
+
+        def free(self):
+            pass
+
+    monkeypatch.setattr(serve._native, "DeviceBuffer", FakeBuffer)
+    pool = serve.DevicePool()
+    assert pool.size_class(1) == 256 and pool.size_class(257) == 512 and pool.size_class(4096) == 4096
+    a, b = pool.take(1000), pool.take(3000)
+    assert (a.nbytes, b.nbytes) == (1024, 4096) and len(made) == 2
+    pool.give(a)
+    pool.give(b)
+    c, d = pool.take(600), pool.take(2100)   # same classes: no new allocation
+    assert c is a and d is b and len(made) == 2
+    e = pool.take(600)                      # none free in that class
+    assert e is not a and len(made) == 3 and pool.allocated == 1024 + 4096 + 1024

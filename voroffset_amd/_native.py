@@ -23,7 +23,7 @@ EXPORTS = ("vo_create", "vo_destroy", "vo_last_error", "vo_morph", "vo_result_sh
            "vo_result_free", "vo_morph_device", "vo_last_stats", "vo_rasterize", "vo_morph_batch_device",
            "vo_last_batch_alone", "vo_measure_device", "vo_xor_device",
            "vo_dexel_mesh", "vo_mesh_shape", "vo_mesh_copy", "vo_mesh_free", "vo_autotune", "vo_tuning_save",
-           "vo_tuning_load", "vo_tuning_get")
+           "vo_tuning_load", "vo_tuning_get", "vo_pass1", "vo_triples_shape", "vo_triples_copy", "vo_triples_free", "vo_copy_device")
 
 
 class VoError(RuntimeError):
@@ -99,6 +99,17 @@ def lib():
         L.vo_tuning_load.restype = i32
         L.vo_tuning_get.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.vo_tuning_get.restype = i32
+        L.vo_pass1.argtypes = [vp, i32, dbl, ctypes.POINTER(VoVolume), ctypes.POINTER(vp)]
+        L.vo_pass1.restype = i32
+        L.vo_triples_shape.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                       ctypes.POINTER(i64)]
+        L.vo_triples_shape.restype = i32
+        L.vo_triples_copy.argtypes = [vp, vp, vp, vp, vp]
+        L.vo_triples_copy.restype = i32
+        L.vo_triples_free.argtypes = [vp]
+        L.vo_triples_free.restype = None
+        L.vo_copy_device.argtypes = [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(i64)]
+        L.vo_copy_device.restype = i32
         L.vo_last_batch_alone.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         L.vo_last_batch_alone.restype = i32
         _lib = L
@@ -112,7 +123,7 @@ class Context:
         self._h = ctypes.c_void_p()
         rc = lib().vo_create(ctypes.byref(self._h), int(device))
         if rc != OK:
-            raise VoError(rc, "vo_create failed")
+            raise VoError(rc, "vo_create failed: " + lib().vo_last_error(None).decode())
 
     def close(self):
         if self._h:
@@ -149,6 +160,27 @@ class Context:
             lib().vo_result_free(res)
         return o, zz, (t1.value, t2.value)
 
+    def pass1_host(self, precision, radius, nx, ny, off, z, z_lo, z_hi):
+        """The pass-1 intermediate of a dilation (vo_pass1): (off, y1, y2, r),
+        CSR over triples per column x + nx*y."""
+        off = np.ascontiguousarray(off, np.int64)
+        z = np.ascontiguousarray(z, np.float64)
+        vol = VoVolume(nx, ny, off.ctypes.data, z.ctypes.data if z.size else None, z_lo, z_hi)
+        res = ctypes.c_void_p()
+        rc = lib().vo_pass1(self._h, PRECISIONS[precision], float(radius), ctypes.byref(vol), ctypes.byref(res))
+        if rc != OK:
+            self._raise(rc)
+        try:
+            rnx, rny, cnt = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
+            lib().vo_triples_shape(res, ctypes.byref(rnx), ctypes.byref(rny), ctypes.byref(cnt))
+            o = np.empty(rnx.value * rny.value + 1, np.int64)
+            y1, y2, r = (np.empty(cnt.value, np.float64) for _ in range(3))
+            p = (lambda a: a.ctypes.data if a.size else None)
+            lib().vo_triples_copy(res, o.ctypes.data, p(y1), p(y2), p(r))
+        finally:
+            lib().vo_triples_free(res)
+        return o, y1, y2, r
+
     def morph_device(self, op, precision, radius, nx, ny, off_ptr, z_ptr, z_lo, z_hi):
         """Device pointers in -> (out VoVolume, count, (time_1, time_2)); the
         output lives in the context's workspace until the next call."""
@@ -178,6 +210,19 @@ class Context:
         if rc != OK:
             self._raise(rc)
         return [(outs[i], cnt[i]) for i in range(n)], (t1.value, t2.value)
+
+    def copy_device(self, copies):
+        """Device-to-device copies [(dst, src, nbytes)] on this context's
+        stream, then one synchronisation (vo_copy_device)."""
+        n = len(copies)
+        if n == 0:
+            return
+        dst = (ctypes.c_void_p * n)(*[c[0] for c in copies])
+        src = (ctypes.c_void_p * n)(*[c[1] for c in copies])
+        nb = (ctypes.c_int64 * n)(*[int(c[2]) for c in copies])
+        rc = lib().vo_copy_device(self._h, n, dst, src, nb)
+        if rc != OK:
+            self._raise(rc)
 
     def rasterize(self, vertices, triangles, nx, ny, origin_x, origin_y, spacing):
         """Mesh -> dexel columns (CSR offsets int64, z float64 in units of spacing)."""
@@ -326,6 +371,30 @@ def device_count():
 
 def synchronize():
     _hip_check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+class DeviceBuffer:
+    """`nbytes` of uninitialised device memory (freed with free() or on GC)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = ctypes.c_void_p()
+        _hip_check(hip().hipMalloc(ctypes.byref(self.ptr), max(self.nbytes, 16)), "hipMalloc")
+
+    @property
+    def address(self):
+        return self.ptr.value
+
+    def free(self):
+        if self.ptr:
+            hip().hipFree(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class DeviceArray:

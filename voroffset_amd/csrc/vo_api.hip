@@ -94,6 +94,7 @@ enum Slot {
   kM_PAIRS, kM_SEGOFF, kM_VERT, kM_QUAD,                     // dexel mesh writer
   kM_SLO, kM_SHI, kM_CHI,                                    // pass-1 block and column summaries
   kX_SCR, kX_N1, kX_N2, kX_LEN, kX_OFF, kX_Z, kX_LENGTH, kX_SEGS, kX_SUM,  // parity metrics
+  kV_WORD, kV_TAB, kV_BASE,                // device-side input check
   kB_OUT,                                  // + (stage * 3 + kind) * 2 (+1 for z): stage outputs
   kSLOTS = kB_OUT + 4 * 3 * 2
 };
@@ -112,8 +113,10 @@ struct vo_context {
   hipStream_t stream = nullptr;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   Buffer buf[kSLOTS];
-  int64_t* host_words = nullptr;  // pinned: status + counts
+  int64_t* host_words = nullptr;  // pinned: status + counts; [3] = input-check words
   std::string error;
+  bool check_input = false;  // an input check was enqueued by this call: every synchronisation looks at it
+  std::string inject;        // VO_INJECT_HIP_ERROR: a HIP call whose label contains it reports an error
   int64_t last_outputs = 0;
   int p1_caps = 96;         // pass-1 slots per column (grows on overflow; a multiple of vo::kSlotBlock)
   int p2_cap_per_col = 16;  // pass-2 arena endpoints per column (grows on overflow)
@@ -121,8 +124,14 @@ struct vo_context {
   int zsweep_events = 0;     // VO_ZSWEEP_EVENTS: force a z-sweep event capacity (512, 768, 1024, 1536)
   Tuning tuning;             // per radius bucket (vo_autotune, vo_tuning_load)
   int pass1_form = 0;        // VO_PASS1: 0 = z-sweep (default), 1 = "paint", 2 = "cull"
+  int axis_forced = -1;      // VO_AXIS: -1 = chosen per volume, 0 = pass 1 along y, 1 = along x
   int last_fallback_tiles = 0;
   std::vector<void*> batch_extra;  // outputs of batched jobs redone alone (freed by the next batch)
+  // host copies of the input-check tables (every call ends synchronised, so
+  // they outlive their uploads)
+  std::vector<char> check_tab;
+  std::vector<int64_t> check_base;
+  std::vector<int> check_jobs;  // the caller's job index of each checked job
   int last_batch_alone = 0;        // jobs of the last batch redone by the single-volume path
 };
 
@@ -130,6 +139,12 @@ struct vo_result {
   int32_t nx = 0, ny = 0;
   std::vector<int64_t> off;
   std::vector<double> z;
+};
+
+struct vo_triples {
+  int32_t nx = 0, ny = 0;
+  std::vector<int64_t> off;  // CSR over triples, column x + nx * y
+  std::vector<double> y1, y2, r;
 };
 
 struct vo_mesh_result {
@@ -150,6 +165,30 @@ void fail(vo_context* ctx, int code, const std::string& msg) {
 
 void check(vo_context* ctx, hipError_t e, const char* what) {
   if (e != hipSuccess) fail(ctx, VO_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+  // fault hook for the error paths of callers (tests): the call itself ran
+  if (!ctx->inject.empty() && std::strstr(what, ctx->inject.c_str()))
+    fail(ctx, VO_ERR_DEVICE, std::string(what) + ": injected HIP error (VO_INJECT_HIP_ERROR)");
+}
+
+// The result of this call's input check (copied to host_words[3] right after
+// the check kernel, so any later synchronisation has it).
+void input_check(vo_context* ctx) {
+  if (!ctx->check_input) return;
+  int w[2];
+  std::memcpy(w, ctx->host_words + 3, sizeof(w));
+  if (w[0] == 0) return;
+  ctx->check_input = false;
+  const int j = w[1] >= 0 && w[1] < int(ctx->check_jobs.size()) ? ctx->check_jobs[w[1]] : w[1];
+  const std::string job = " (job " + std::to_string(j) + ")";
+  if (w[0] & vo::kInputMalformed)
+    fail(ctx, VO_ERR_ARGUMENT, "input volume: decreasing offsets, odd column, unsorted or non-finite endpoint" + job);
+  fail(ctx, VO_ERR_RANGE, "input volume: endpoint outside the z-range the operation needs "
+                          "([z_lo - 1, z_hi + 1]; for a closing also after dilation by the radius)" + job);
+}
+
+void sync(vo_context* ctx) {
+  check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+  input_check(ctx);
 }
 
 void* ws(vo_context* ctx, int slot, size_t bytes) {
@@ -168,6 +207,9 @@ void* ws(vo_context* ctx, int slot, size_t bytes) {
   return b.ptr;
 }
 
+// The buffer ws() last gave for `slot`.
+void* slot_ptr(vo_context* ctx, int slot) { return ctx->buf[slot].ptr; }
+
 unsigned blocks_for(int64_t n, int threads) { return unsigned((n + threads - 1) / threads); }
 
 // Exclusive scan of len[0..n] (len[n] must be 0) into off[0..n].
@@ -178,19 +220,10 @@ void exclusive_scan(vo_context* ctx, const int64_t* len, int64_t* off, int64_t n
   check(ctx, hipcub::DeviceScan::ExclusiveSum(t, tmp, len, off, int(n), ctx->stream), "scan");
 }
 
-int32_t read_word32(vo_context* ctx, const int* dptr) {
-  check(ctx, hipMemcpyAsync(ctx->host_words + 2, dptr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
-        "copy word");
-  check(ctx, hipStreamSynchronize(ctx->stream), "sync");
-  int32_t v;
-  std::memcpy(&v, ctx->host_words + 2, sizeof(v));
-  return v;
-}
-
 int64_t read_word(vo_context* ctx, const int64_t* dptr) {
   check(ctx, hipMemcpyAsync(ctx->host_words, dptr, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream),
         "copy word");
-  check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+  sync(ctx);
   return ctx->host_words[0];
 }
 
@@ -291,11 +324,34 @@ int ocap_for(const vo_context* ctx, int rint) {
   return rint < 6 ? std::max(24, o) : o;
 }
 
+// The logical grid the passes work on.  Pass 1 sweeps along the ny
+// positions of each of the nx lines, pass 2 across the lines; the CSR index
+// of (line x, position y) is x * sx + y * sy.  Along y (the reference's
+// axis, Voronoi2D.cpp sweeps each x-line) the logical grid is the physical
+// one; along x it is transposed, and a pass-1 window of consecutive
+// positions is then one contiguous range of the CSR.  Along x is used
+// unless the rows are so short that 64-column tiles along them fill
+// noticeably worse than tiles along y (VO_AXIS=x|y forces one).
+struct Axis {
+  int nx, ny;
+  int64_t sx, sy;
+};
+
+Axis axis_for(const vo_context* ctx, int nx, int ny) {
+  auto fill = [](int n) { return n <= 0 ? 1.0 : double(n) / (64.0 * ((n + 63) / 64)); };
+  const bool along_x = ctx->axis_forced >= 0 ? ctx->axis_forced == 1 : fill(nx) >= 0.75 * fill(ny);
+  if (along_x) return Axis{ny, nx, int64_t(nx), 1};
+  return Axis{nx, ny, 1, int64_t(nx)};
+}
+
 // One dilation: `in` -> CSR in (out_off_slot, out_z_slot).
 template <class T>
 DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int out_z_slot, double* t1,
               double* t2) {
-  const int nx = in.nx, ny = in.ny;
+  // the passes work on a logical grid of nx lines of ny positions: pass 1
+  // sweeps along the positions of each line, pass 2 across the lines
+  const Axis ax = axis_for(ctx, in.nx, in.ny);
+  const int nx = ax.nx, ny = ax.ny;
   const int64_t ncols = int64_t(nx) * ny;
   if (!(R >= 0) || !std::isfinite(R)) fail(ctx, VO_ERR_ARGUMENT, "radius must be finite and >= 0");
   if (R > 60000) fail(ctx, VO_ERR_ARGUMENT, "radius too large");
@@ -307,15 +363,15 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
   int64_t* o_len = static_cast<int64_t*>(ws(ctx, kO_LEN, (ncols + 1) * sizeof(int64_t)));
   int64_t* out_off = static_cast<int64_t*>(ws(ctx, out_off_slot, (ncols + 1) * sizeof(int64_t)));
   DevVol out;
-  out.nx = nx;
-  out.ny = ny;
+  out.nx = in.nx;
+  out.ny = in.ny;
   out.off = out_off;
   if (ncols == 0) {
     check(ctx, hipMemsetAsync(out_off, 0, sizeof(int64_t), ctx->stream), "memset");
     out.z = ws(ctx, out_z_slot, 16);
     return out;
   }
-  std::vector<double> htab_host;  // outlives the uploads (read_word32 below synchronises)
+  std::vector<double> htab_host;  // outlives the uploads (the gather is followed by a synchronisation)
   int chunk1 = std::min(Shapes<T>::p1_cmax, 32);
   int chunk2 = Shapes<T>::p2_cmax;
   bool zsweep_failed = false;  // a window too dense for the z-sweep form: sweep along y instead
@@ -354,13 +410,13 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
       fail(ctx, VO_ERR_ARGUMENT, "grid too large");
 
     vo::Pass1Args<T> a1{nx, ny, rint, chunk1, nch1, in.off, static_cast<const T*>(in.z), m_lo, m_hi, m_dy,
-                        caps, m_cnt, status};
+                        caps, m_cnt, status, ax.sx, ax.sy};
     check(ctx, hipEventRecord(ctx->ev[0], ctx->stream), "event");
     const bool zsweep = ctx->pass1_form == 0 && rint <= 127 && !zsweep_failed &&
                         vo::zsweep_offsets_fit(caps, ny, int(sizeof(T)));
     if (zsweep) {
       vo::Pass1ZArgs<T> az{nx, ny, rint, in.off, static_cast<const T*>(in.z), m_lo, m_hi, m_dy, caps, m_cnt,
-                           status, s_lo, s_hi, c_hi};
+                           status, s_lo, s_hi, c_hi, ax.sx, ax.sy};
       const int64_t gridz = int64_t(nx) * ((ny + vo::kZTile - 1) / vo::kZTile);
       launch_zsweep_events<T>(ctx, events_for(ctx, rint), rint, gridz, az);
     } else if (ctx->pass1_form == 2) {
@@ -380,7 +436,8 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
             static_cast<double*>(ws(ctx, kG_LISTS, size_t(ntiles) * 2 * gocap * vo::kWave * sizeof(double)));
         // the summaries exist only when the z-sweep form wrote pass 1
         vo::Pass2GArgs<T> ag{nx, ny, rint, R2, m_lo, m_hi, m_dy, caps, m_cnt, o_z, garena, o_start, o_len,
-                             tiles, status, gocap, ht, 0, lists, zsweep ? s_lo : nullptr, zsweep ? s_hi : nullptr, c_hi};
+                             tiles, status, gocap, ht, 0, lists, zsweep ? s_lo : nullptr, zsweep ? s_hi : nullptr, c_hi,
+                             ax.sx, ax.sy};
         if (rint <= vo::kGatherLdsTableMax)
           hipLaunchKernelGGL((vo::pass2_gather_kernel<T, true>), dim3(unsigned(ntiles)), dim3(vo::kWave),
                              size_t(rint + 1) * (rint + 1) * sizeof(double), ctx->stream, ag);
@@ -401,7 +458,7 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
         check(ctx, hipMemcpyAsync(ctx->host_words, out_off + ncols, sizeof(int64_t), hipMemcpyDeviceToHost,
                                   ctx->stream),
               "copy word");
-        check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+        sync(ctx);
         std::memcpy(&nfall, ctx->host_words + 2, sizeof(int));
         if (nfall == 0 && int(ctx->host_words[1] & 0xffffffff) == 0) fast_total = ctx->host_words[0];
         if (nfall == 0 || gocap >= 64) break;
@@ -416,7 +473,7 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
     const int64_t grid2 = gather ? int64_t(nfall) * vo::kGatherTile : sweep_blocks;
     if (grid2 > 0) {
       vo::Pass2Args<T> a2{nx, ny, rint, R2, chunk2_eff, 0, m_lo, m_hi, m_dy, caps, m_cnt, o_z, arena2, o_start,
-                          o_len, status, gather ? tiles : nullptr, ntiles * garena};
+                          o_len, status, gather ? tiles : nullptr, ntiles * garena, ax.sx, ax.sy};
       hipLaunchKernelGGL((vo::pass2_kernel<T, kP2SCap, kP2PCap, kP2NCap, kP2OCap, Shapes<T>::p2_farena,
                                            Shapes<T>::p2_cmax>),
                          dim3(unsigned(grid2)), dim3(vo::kWave), 0, ctx->stream, a2);
@@ -437,7 +494,7 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
     check(ctx, hipMemsetAsync(o_len + ncols, 0, sizeof(int64_t), ctx->stream), "memset");
     check(ctx, hipMemcpyAsync(ctx->host_words + 1, status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
           "copy status");
-    check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    sync(ctx);
     const int st = int(ctx->host_words[1] & 0xffffffff);
     if (st == 0) {
       if (t1 || t2) {
@@ -471,12 +528,69 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
   }
   T* oz = static_cast<T*>(ws(ctx, out_z_slot, size_t(std::max<int64_t>(total, 1)) * sizeof(T)));
   hipLaunchKernelGGL(vo::gather_kernel<T>, dim3(blocks_for(ncols, 256)), dim3(256), 0, ctx->stream, ncols,
-                     static_cast<const T*>(ctx->buf[kO_Z].ptr), o_start, o_len, out_off, oz);
+                     static_cast<const T*>(slot_ptr(ctx, kO_Z)), o_start, o_len, out_off, oz);
   check(ctx, hipGetLastError(), "gather launch");
   out.z = oz;
   out.count = total;
   ctx->last_outputs = total;
   return out;
+}
+
+// Device-side input check of the resident volumes of this call (device
+// entry points; the host-buffer entry point checks on the host).  Nothing
+// waits for it here: the result is copied to host_words[3] and looked at by
+// every later synchronisation of the call (sync()), before anything else it
+// reads, so a bad input fails with VO_ERR_ARGUMENT / VO_ERR_RANGE.  The
+// kernels of the operation are memory-safe on any input whose offsets index
+// the endpoint buffer, so they may run meanwhile.
+template <class T>
+vo::InputCheck<T> input_check_of(int op, double R, const vo_volume& v) {
+  vo::InputCheck<T> c{v.offsets, static_cast<const T*>(v.z), 0, 0.0, 0.0};
+  if (op == VO_OP_EROSION || op == VO_OP_OPENING) {  // Voronoi.cpp:27-42: the complement's ring range
+    c.range = 1;
+    c.lo = v.z_lo - 1;
+    c.hi = v.z_hi + 1;
+  } else if (op == VO_OP_CLOSING) {  // the dilation first, then the same range
+    c.range = 1;
+    c.lo = v.z_lo - 1 + R;
+    c.hi = v.z_hi + 1 - R;
+  }
+  return c;
+}
+
+template <class T>
+void enqueue_input_check(vo_context* ctx, const std::vector<vo::InputCheck<T>>& jobs,
+                         const std::vector<int64_t>& ncols, const std::vector<int>& job_ids) {
+  const int K = int(jobs.size());
+  ctx->check_jobs = job_ids;
+  int* word = static_cast<int*>(ws(ctx, kV_WORD, 2 * sizeof(int)));
+  check(ctx, hipMemsetAsync(word, 0, sizeof(int), ctx->stream), "memset");
+  check(ctx, hipMemsetAsync(word + 1, 0x7f, sizeof(int), ctx->stream), "memset");  // a large job index
+  ctx->check_base.assign(size_t(K) + 1, 0);
+  for (int k = 0; k < K; ++k) ctx->check_base[k + 1] = ctx->check_base[k] + ncols[k];
+  const int64_t total = ctx->check_base[K];
+  if (total > INT32_MAX) fail(ctx, VO_ERR_ARGUMENT, "batch too large");
+  if (total > 0) {
+    const dim3 grid(blocks_for(total, 256));
+    if (K == 1) {
+      hipLaunchKernelGGL(vo::check_input_kernel<T>, grid, dim3(256), 0, ctx->stream, jobs[0], total, word);
+    } else {
+      const size_t tb = size_t(K) * sizeof(vo::InputCheck<T>);
+      ctx->check_tab.resize(tb);
+      std::memcpy(ctx->check_tab.data(), jobs.data(), tb);
+      auto* dtab = static_cast<vo::InputCheck<T>*>(ws(ctx, kV_TAB, tb));
+      auto* dbase = static_cast<int64_t*>(ws(ctx, kV_BASE, size_t(K + 1) * sizeof(int64_t)));
+      check(ctx, hipMemcpyAsync(dtab, ctx->check_tab.data(), tb, hipMemcpyHostToDevice, ctx->stream), "upload");
+      check(ctx, hipMemcpyAsync(dbase, ctx->check_base.data(), size_t(K + 1) * sizeof(int64_t),
+                                hipMemcpyHostToDevice, ctx->stream),
+            "upload");
+      hipLaunchKernelGGL(vo::check_input_batch_kernel<T>, grid, dim3(256), 0, ctx->stream, dtab, dbase, K, word);
+    }
+    check(ctx, hipGetLastError(), "input check launch");
+  }
+  check(ctx, hipMemcpyAsync(ctx->host_words + 3, word, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
+        "copy input check");
+  ctx->check_input = true;
 }
 
 // Per-column complement kernels (phase 0: lengths, phase 1: data): the ring
@@ -591,7 +705,7 @@ void dilate_batch(vo_context* ctx, std::vector<DevVol>& vol, const std::vector<d
     if (!(R[j] >= 0) || !std::isfinite(R[j]) || R[j] > 60000) fail(ctx, VO_ERR_ARGUMENT, "radius must be finite and >= 0");
     const int rint = int(std::floor(R[j]));
     if (rint > 127 || int64_t(vol[j].nx) * vol[j].ny == 0 || ctx->force_sweep || ctx->pass1_form != 0 ||
-        !vo::zsweep_offsets_fit(ctx->p1_caps, vol[j].ny, int(sizeof(T))))
+        !vo::zsweep_offsets_fit(ctx->p1_caps, axis_for(ctx, vol[j].nx, vol[j].ny).ny, int(sizeof(T))))
       alone.push_back(j);
     else
       jobs.push_back(j);
@@ -610,7 +724,8 @@ void dilate_batch(vo_context* ctx, std::vector<DevVol>& vol, const std::vector<d
       const DevVol& v = vol[jobs[k]];
       const int rint = int(std::floor(R[jobs[k]]));
       col_base[k + 1] = col_base[k] + int64_t(v.nx) * v.ny;
-      tile_base[k + 1] = tile_base[k] + int64_t(v.nx) * ((v.ny + vo::kGatherTile - 1) / vo::kGatherTile);
+      const Axis ax = axis_for(ctx, v.nx, v.ny);
+      tile_base[k + 1] = tile_base[k] + int64_t(ax.nx) * ((ax.ny + vo::kGatherTile - 1) / vo::kGatherTile);
       ocap = std::max(ocap, ocap_for(ctx, rint));
     }
     std::vector<double> radii(K);
@@ -645,12 +760,13 @@ void dilate_batch(vo_context* ctx, std::vector<DevVol>& vol, const std::vector<d
       const int rint = int(std::floor(R[jobs[k]]));
       const int64_t cb = col_base[k], sb = cb * caps;
       const int64_t qb = cb * (caps / vo::kSlotBlock);  // first block summary of this volume
-      a1[k] = vo::Pass1ZArgs<T>{v.nx, v.ny, rint, v.off, static_cast<const T*>(v.z), m_lo + sb, m_hi + sb, m_dy + sb,
-                                caps, m_cnt + cb, status + k, s_lo + qb, s_hi + qb, c_hi + cb};
-      a2[k] = vo::Pass2GArgs<T>{v.nx, v.ny, rint, R[jobs[k]] * R[jobs[k]], m_lo + sb, m_hi + sb, m_dy + sb, caps,
+      const Axis ax = axis_for(ctx, v.nx, v.ny);
+      a1[k] = vo::Pass1ZArgs<T>{ax.nx, ax.ny, rint, v.off, static_cast<const T*>(v.z), m_lo + sb, m_hi + sb, m_dy + sb,
+                                caps, m_cnt + cb, status + k, s_lo + qb, s_hi + qb, c_hi + cb, ax.sx, ax.sy};
+      a2[k] = vo::Pass2GArgs<T>{ax.nx, ax.ny, rint, R[jobs[k]] * R[jobs[k]], m_lo + sb, m_hi + sb, m_dy + sb, caps,
                                 m_cnt + cb, o_z, garena, o_start + cb, o_len + cb, nullptr, status + k, ocap,
                                 htab + htab_base[k], tile_base[k] * garena, lists + tile_base[k] * list_tile,
-                                s_lo + qb, s_hi + qb, c_hi + cb};
+                                s_lo + qb, s_hi + qb, c_hi + cb, ax.sx, ax.sy};
     }
     const auto* d1 = upload_table(ctx, kB_TAB0, a1);
     const auto* d2 = upload_table(ctx, kB_TAB1, a2);
@@ -770,7 +886,7 @@ void complement_batch(vo_context* ctx, bool ring, std::vector<DevVol>& vol, cons
   for (auto& e : t) e.out = z;
   dt = upload_table(ctx, kB_TAB0, t);
   launch(1);
-  check(ctx, hipStreamSynchronize(ctx->stream), "sync");  // the host table must outlive its upload
+  sync(ctx);  // the host table must outlive its upload
   for (int k = 0; k < K; ++k) {
     DevVol& v = vol[idx[k]];
     v.nx = ring ? v.nx + 2 : v.nx - 2;
@@ -850,6 +966,18 @@ void morph_batch_device(vo_context* ctx, int n, const vo_job* jobs, vo_volume* o
   }
   if (t1) *t1 = 0;
   if (t2) *t2 = 0;
+  {
+    std::vector<vo::InputCheck<T>> chk;
+    std::vector<int64_t> nc;
+    std::vector<int> ids;
+    for (int j = 0; j < n; ++j) {
+      if (jobs[j].op == VO_OP_NOOP) continue;  // passed through untouched
+      chk.push_back(input_check_of<T>(jobs[j].op, jobs[j].radius, jobs[j].in));
+      nc.push_back(int64_t(jobs[j].in.nx) * jobs[j].in.ny);
+      ids.push_back(j);
+    }
+    if (!chk.empty()) enqueue_input_check<T>(ctx, chk, nc, ids);
+  }
   run_batch<T>(ctx, op, R, vol, zlo, zhi, t1, t2);
   if (n == 0) return;
   // rebase every result's offsets to 0 in one buffer
@@ -869,7 +997,7 @@ void morph_batch_device(vo_context* ctx, int n, const vo_job* jobs, vo_volume* o
   check(ctx, hipGetLastError(), "finalize launch");
   std::vector<int64_t> h(size_t(2 * n));
   check(ctx, hipMemcpyAsync(h.data(), aux, h.size() * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream), "copy");
-  check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+  sync(ctx);
   for (int j = 0; j < n; ++j) {
     out[j].nx = vol[j].nx;
     out[j].ny = vol[j].ny;
@@ -1075,13 +1203,18 @@ void autotune(vo_context* ctx, const vo_volume* sample, int n, const double* rad
 
 int guarded(vo_context* ctx, const std::function<void()>& body) {
   if (!ctx) return VO_ERR_ARGUMENT;
+  ctx->check_input = false;
+  (void)hipGetLastError();  // an error left by an earlier, already reported call is not this call's
   try {
     check(ctx, hipSetDevice(ctx->device), "hipSetDevice");
     body();
+    ctx->check_input = false;
     return VO_OK;
   } catch (const Failure& f) {
+    ctx->check_input = false;
     return f.code;
   } catch (const std::exception& e) {
+    ctx->check_input = false;
     ctx->error = e.what();
     return VO_ERR_ARGUMENT;
   }
@@ -1181,16 +1314,19 @@ void rasterize(vo_context* ctx, const double* V, int64_t nv, const int32_t* F, i
   *out = res.release();
 }
 
+// FP32 travels relative to an integer shift of the z-range centre.
 template <class T>
-void morph_host(vo_context* ctx, int op, double R, const vo_volume* in, vo_result** out, double* t1,
-                double* t2) {
-  validate_shape(ctx, in);
-  validate_host(ctx, in, op, R);
+double storage_shift(const vo_volume* in) {
+  return std::is_same<T, float>::value ? std::round(0.5 * (in->z_lo + in->z_hi)) : 0.0;
+}
+
+// A checked host volume (double z) uploaded in storage precision T, minus
+// `shift`.  `zt` holds the converted endpoints until the upload has run.
+template <class T>
+DevVol upload_host(vo_context* ctx, const vo_volume* in, double shift, std::vector<T>& zt) {
   const int64_t n = int64_t(in->nx) * in->ny;
   const int64_t cnt = n ? in->offsets[n] : 0;
-  // FP32 travels relative to an integer shift of the z-range centre
-  const double shift = std::is_same<T, float>::value ? std::round(0.5 * (in->z_lo + in->z_hi)) : 0.0;
-  std::vector<T> zt(size_t(std::max<int64_t>(cnt, 1)));
+  zt.assign(size_t(std::max<int64_t>(cnt, 1)), T(0));
   const double* z = static_cast<const double*>(in->z);
   for (int64_t i = 0; i < cnt; ++i) zt[i] = T(z[i] - shift);
   int64_t* doff = static_cast<int64_t*>(ws(ctx, kIN_OFF, (n + 1) * sizeof(int64_t)));
@@ -1204,6 +1340,17 @@ void morph_host(vo_context* ctx, int op, double R, const vo_volume* in, vo_resul
   v.off = doff;
   v.z = dz;
   v.count = cnt;
+  return v;
+}
+
+template <class T>
+void morph_host(vo_context* ctx, int op, double R, const vo_volume* in, vo_result** out, double* t1,
+                double* t2) {
+  validate_shape(ctx, in);
+  validate_host(ctx, in, op, R);
+  const double shift = storage_shift<T>(in);
+  std::vector<T> zt;
+  DevVol v = upload_host<T>(ctx, in, shift, zt);
   DevVol r = v;
   if (op != VO_OP_NOOP) r = run_op<T>(ctx, op, v, R, in->z_lo - shift, in->z_hi - shift, t1, t2);
   auto* res = new vo_result;
@@ -1221,6 +1368,67 @@ void morph_host(vo_context* ctx, int op, double R, const vo_volume* in, vo_resul
   *out = res;
 }
 
+// The pass-1 intermediate of a dilation of a host volume, as CSR triples.
+// The dilation runs whole; its pass-1 buffers stay in the workspace and are
+// exported from there.
+template <class T>
+void pass1_host(vo_context* ctx, double R, const vo_volume* in, vo_triples* res) {
+  validate_shape(ctx, in);
+  validate_host(ctx, in, VO_OP_DILATION, R);
+  const double shift = storage_shift<T>(in);
+  std::vector<T> zt;
+  DevVol v = upload_host<T>(ctx, in, shift, zt);
+  // the reference's axis: lines of constant x, swept along y
+  const int forced = ctx->axis_forced;
+  ctx->axis_forced = 0;
+  try {
+    dilate<T>(ctx, v, R, kV3_OFF, kV3_Z, nullptr, nullptr);
+  } catch (...) {
+    ctx->axis_forced = forced;
+    throw;
+  }
+  ctx->axis_forced = forced;
+  const int64_t n = int64_t(in->nx) * in->ny;
+  res->nx = in->nx;
+  res->ny = in->ny;
+  res->off.assign(size_t(n) + 1, 0);
+  if (n == 0) return;
+  int64_t* len = static_cast<int64_t*>(ws(ctx, kX_LEN, size_t(n + 1) * sizeof(int64_t)));
+  int64_t* off = static_cast<int64_t*>(ws(ctx, kX_OFF, size_t(n + 1) * sizeof(int64_t)));
+  vo::TripleExport<T> g{in->nx, in->ny, ctx->p1_caps, static_cast<const T*>(slot_ptr(ctx, kM_LO)),
+                        static_cast<const T*>(slot_ptr(ctx, kM_HI)), static_cast<const uint16_t*>(slot_ptr(ctx, kM_DY)),
+                        static_cast<const int32_t*>(slot_ptr(ctx, kM_CNT)), R * R, shift, len, off, nullptr, nullptr,
+                        nullptr};
+  const dim3 grid(blocks_for(n, 256));
+  hipLaunchKernelGGL(vo::export_triples_kernel<T>, grid, dim3(256), 0, ctx->stream, g, 0);
+  check(ctx, hipGetLastError(), "export launch");
+  check(ctx, hipMemsetAsync(len + n, 0, sizeof(int64_t), ctx->stream), "memset");
+  exclusive_scan(ctx, len, off, n + 1);
+  const int64_t total = read_word(ctx, off + n);
+  const int64_t stride = std::max<int64_t>(total, 1);
+  double* y = static_cast<double*>(ws(ctx, kX_SCR, size_t(stride) * 3 * sizeof(double)));
+  g.y1 = y;
+  g.y2 = y + stride;
+  g.r = y + 2 * stride;
+  hipLaunchKernelGGL(vo::export_triples_kernel<T>, grid, dim3(256), 0, ctx->stream, g, 1);
+  check(ctx, hipGetLastError(), "export launch");
+  res->y1.resize(size_t(total));
+  res->y2.resize(size_t(total));
+  res->r.resize(size_t(total));
+  check(ctx, hipMemcpyAsync(res->off.data(), off, size_t(n + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream),
+        "download");
+  if (total) {
+    const size_t bytes = size_t(total) * sizeof(double);
+    check(ctx, hipMemcpyAsync(res->y1.data(), g.y1, bytes, hipMemcpyDeviceToHost, ctx->stream), "download");
+    check(ctx, hipMemcpyAsync(res->y2.data(), g.y2, bytes, hipMemcpyDeviceToHost, ctx->stream), "download");
+    check(ctx, hipMemcpyAsync(res->r.data(), g.r, bytes, hipMemcpyDeviceToHost, ctx->stream), "download");
+  }
+  sync(ctx);
+}
+
+// Why this thread's last vo_create failed (vo_last_error(NULL)).
+thread_local std::string g_create_error;
+
 }  // namespace
 
 extern "C" {
@@ -1232,6 +1440,8 @@ int vo_create(vo_context** out, int device) {
   ctx->device = device;
   if (const char* e = std::getenv("VO_PASS2")) ctx->force_sweep = std::string(e) == "sweep";
   if (const char* e = std::getenv("VO_ZSWEEP_EVENTS")) ctx->zsweep_events = std::atoi(e);
+  if (const char* e = std::getenv("VO_INJECT_HIP_ERROR")) ctx->inject = e;
+  if (const char* e = std::getenv("VO_AXIS")) ctx->axis_forced = std::string(e) == "x" ? 1 : (std::string(e) == "y" ? 0 : -1);
   if (const char* e = std::getenv("VO_PASS1"))
     ctx->pass1_form = std::string(e) == "cull" ? 2 : (std::string(e) == "paint" ? 1 : 0);
   int rc = guarded(ctx, [&] {
@@ -1240,15 +1450,16 @@ int vo_create(vo_context** out, int device) {
     if (device < 0 || device >= n) fail(ctx, VO_ERR_ARGUMENT, "no such HIP device");
     check(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "stream");
     for (auto& e : ctx->ev) check(ctx, hipEventCreate(&e), "event");
-    check(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->host_words), 4 * sizeof(int64_t), hipHostMallocDefault),
+    check(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->host_words), 8 * sizeof(int64_t), hipHostMallocDefault),
           "pinned");
   });
   if (rc != VO_OK) {
-    static thread_local std::string last;
-    last = ctx->error;
+    g_create_error = ctx->error;
     vo_destroy(ctx);
+    (void)hipGetLastError();  // reported here, not by the next launch check of another context
     return rc;
   }
+  g_create_error.clear();
   *out = ctx;
   return VO_OK;
 }
@@ -1267,7 +1478,10 @@ void vo_destroy(vo_context* ctx) {
   delete ctx;
 }
 
-const char* vo_last_error(const vo_context* ctx) { return ctx ? ctx->error.c_str() : "no context"; }
+const char* vo_last_error(const vo_context* ctx) {
+  if (ctx) return ctx->error.c_str();
+  return g_create_error.empty() ? "no context" : g_create_error.c_str();
+}
 
 int vo_morph(vo_context* ctx, int op, int precision, double radius, const vo_volume* in, vo_result** out,
              double* time_1_ms, double* time_2_ms) {
@@ -1300,6 +1514,55 @@ int vo_result_copy(const vo_result* r, int64_t* offsets, double* z) {
 
 void vo_result_free(vo_result* r) { delete r; }
 
+int vo_pass1(vo_context* ctx, int precision, double radius, const vo_volume* in, vo_triples** out) {
+  if (!out) return VO_ERR_ARGUMENT;
+  *out = nullptr;
+  std::unique_ptr<vo_triples> t(new vo_triples);
+  const int rc = guarded(ctx, [&] {
+    if (precision == VO_FP32)
+      pass1_host<float>(ctx, radius, in, t.get());
+    else if (precision == VO_FP64)
+      pass1_host<double>(ctx, radius, in, t.get());
+    else
+      fail(ctx, VO_ERR_ARGUMENT, "unknown precision");
+  });
+  if (rc == VO_OK) *out = t.release();
+  return rc;
+}
+
+int vo_triples_shape(const vo_triples* t, int32_t* nx, int32_t* ny, int64_t* count) {
+  if (!t) return VO_ERR_ARGUMENT;
+  if (nx) *nx = t->nx;
+  if (ny) *ny = t->ny;
+  if (count) *count = int64_t(t->y1.size());
+  return VO_OK;
+}
+
+int vo_triples_copy(const vo_triples* t, int64_t* offsets, double* y1, double* y2, double* r) {
+  if (!t) return VO_ERR_ARGUMENT;
+  if (offsets) std::memcpy(offsets, t->off.data(), t->off.size() * sizeof(int64_t));
+  const size_t n = t->y1.size() * sizeof(double);
+  if (n && y1) std::memcpy(y1, t->y1.data(), n);
+  if (n && y2) std::memcpy(y2, t->y2.data(), n);
+  if (n && r) std::memcpy(r, t->r.data(), n);
+  return VO_OK;
+}
+
+void vo_triples_free(vo_triples* t) { delete t; }
+
+int vo_copy_device(vo_context* ctx, int n, void* const* dst, const void* const* src, const int64_t* bytes) {
+  return guarded(ctx, [&] {
+    if (n < 0 || (n > 0 && (!dst || !src || !bytes))) fail(ctx, VO_ERR_ARGUMENT, "vo_copy_device: bad arguments");
+    for (int i = 0; i < n; ++i) {
+      if (bytes[i] < 0) fail(ctx, VO_ERR_ARGUMENT, "vo_copy_device: negative size");
+      if (bytes[i] > 0)
+        check(ctx, hipMemcpyAsync(dst[i], src[i], size_t(bytes[i]), hipMemcpyDeviceToDevice, ctx->stream), "copy");
+    }
+    sync(ctx);
+  });
+}
+
+
 int vo_rasterize(vo_context* ctx, const double* vertices, int64_t num_vertices, const int32_t* triangles,
                  int64_t num_triangles, int32_t nx, int32_t ny, double origin_x, double origin_y, double spacing,
                  vo_result** out) {
@@ -1330,13 +1593,17 @@ int vo_morph_device(vo_context* ctx, int op, int precision, double radius, const
       const int64_t n = int64_t(in->nx) * in->ny;
       r.count = n ? read_word(ctx, in->offsets + n) : 0;
     } else if (precision == VO_FP32) {
+      if (op < VO_OP_NOOP || op > VO_OP_OPENING) fail(ctx, VO_ERR_ARGUMENT, "unknown operation");
+      enqueue_input_check<float>(ctx, {input_check_of<float>(op, radius, *in)}, {int64_t(in->nx) * in->ny}, {0});
       r = run_op<float>(ctx, op, v, radius, in->z_lo, in->z_hi, time_1_ms, time_2_ms);
     } else if (precision == VO_FP64) {
+      if (op < VO_OP_NOOP || op > VO_OP_OPENING) fail(ctx, VO_ERR_ARGUMENT, "unknown operation");
+      enqueue_input_check<double>(ctx, {input_check_of<double>(op, radius, *in)}, {int64_t(in->nx) * in->ny}, {0});
       r = run_op<double>(ctx, op, v, radius, in->z_lo, in->z_hi, time_1_ms, time_2_ms);
     } else {
       fail(ctx, VO_ERR_ARGUMENT, "unknown precision");
     }
-    check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    sync(ctx);
     out->nx = r.nx;
     out->ny = r.ny;
     out->offsets = r.off;

@@ -54,6 +54,10 @@ struct Pass1Args {
   int caps;        // slots per column
   int32_t* m_cnt;  // per column, at count_index: number of triples
   int* status;
+  // CSR index of the column at (line x, position y) of this logical grid:
+  // x * sx + y * sy (sx = 1, sy = nx sweeps along y, the reference's axis;
+  // sx = ny, sy = 1 with nx, ny swapped sweeps along x)
+  int64_t sx, sy;
 };
 
 // Slot layout of the pass-1 triples: blocks of kSlotBlock consecutive
@@ -207,7 +211,7 @@ __global__ __launch_bounds__(kWave) void pass1_kernel(Pass1Args<T> g) {
   int cur = 0, n = 0, apos = 0;
   bool ok = true;
   for (int y = max(0, y0 - g.rint); y < y1 && ok; ++y) {
-    const int64_t col = int64_t(x) + int64_t(g.nx) * y;
+    const int64_t col = int64_t(x) * g.sx + int64_t(y) * g.sy;
     const int64_t zo = g.in_off[col], zl = g.in_off[col + 1] - zo;
     n = pass1_step<T>(sm, cur, n, y, g.rint, g.in_z, zo, zl, NCAP, CAP, g.status);
     if (n < 0) {
@@ -239,7 +243,7 @@ __global__ __launch_bounds__(kWave) void pass1_kernel(Pass1Args<T> g) {
   n = 0;
   for (int y = min(g.ny - 1, y1 - 1 + g.rint); y >= y0; --y) {
     const int k = g.ny - 1 - y;
-    const int64_t col = int64_t(x) + int64_t(g.nx) * y;
+    const int64_t col = int64_t(x) * g.sx + int64_t(y) * g.sy;
     const int64_t zo = g.in_off[col], zl = g.in_off[col + 1] - zo;
     n = pass1_step<T>(sm, cur, n, k, g.rint, g.in_z, zo, zl, NCAP, CAP, g.status);
     if (n < 0) return;
@@ -293,6 +297,10 @@ struct Pass2Args {
   // arenas from o_base on
   const int* tiles;
   int64_t o_base;
+  // CSR index of the column at (line x, position y) of this logical grid:
+  // x * sx + y * sy (sx = 1, sy = nx sweeps along y, the reference's axis;
+  // sx = ny, sy = 1 with nx, ny swapped sweeps along x)
+  int64_t sx, sy;
 };
 
 template <class T, int SCAP, int PCAP, int NCAP, int OCAP, int FARENA, int CMAX>
@@ -705,7 +713,7 @@ __global__ __launch_bounds__(kWave) void pass2_kernel(Pass2Args<T> g) {
   int64_t opos = 0;
   for (int x = min(g.nx - 1, x1 - 1 + g.rint); x >= x0; --x) {
     const int k = g.nx - 1 - x;
-    const int64_t col = int64_t(x) + int64_t(g.nx) * y;
+    const int64_t col = int64_t(x) * g.sx + int64_t(y) * g.sy;
     const int nu = pass2_step<T>(sm, cs, ns, cp, np, k, x, y, g.m_cnt[count_index(g.ny, x, y)], g, NCAP, SCAP, PCAP, OCAP,
                                  sm.ulo[2], sm.uhi[2]);
     if (nu < 0) return;
@@ -743,6 +751,102 @@ __global__ void gather_kernel(int64_t ncols, const T* src, const int64_t* start,
   const int64_t s = start[c], o = off[c];
   const int64_t n = cnt[c];
   for (int64_t i = 0; i < n; ++i) dst[o + i] = src[s + i];
+}
+
+// Device-side check of the CompressedVolume invariants the kernels rely on
+// (the host-buffer entry point checks the same on the host): offsets
+// non-decreasing with even column lengths, endpoints finite and sorted, and,
+// for the operations that complement within a z-range, the first and last
+// endpoint of every column inside [lo, hi] (the caller widens or narrows the
+// range for the operation).  The reference asserts these with vor_assert
+// (src/vor3d/Common.h:228, Common.cpp:8-18).  word[0] collects the bits,
+// word[1] the smallest failing job.
+template <class T>
+struct InputCheck {
+  const int64_t* off;
+  const T* z;
+  int range;       // 1: check [lo, hi]
+  double lo, hi;
+};
+
+template <class T>
+__device__ void check_column(const InputCheck<T>& g, int64_t c, int job, int* word) {
+  const int64_t s = g.off[c], e = g.off[c + 1];
+  int bad = 0;
+  if (e < s || ((e - s) & 1)) {
+    bad = kInputMalformed;
+  } else if (e > s) {
+    T prev = g.z[s];
+    int ok = isfinite(double(prev)) ? 1 : 0;
+    for (int64_t i = s + 1; i < e; ++i) {
+      const T v = g.z[i];
+      ok = (ok && isfinite(double(v)) && v >= prev) ? 1 : 0;
+      prev = v;
+    }
+    if (!ok)
+      bad = kInputMalformed;
+    else if (g.range && (double(g.z[s]) < g.lo || double(prev) > g.hi))
+      bad = kInputOutOfRange;
+  }
+  if (bad) {
+    atomicOr(word, bad);
+    atomicMin(word + 1, job);
+  }
+}
+
+template <class T>
+__global__ void check_input_kernel(InputCheck<T> g, int64_t ncols, int* word) {
+  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c < ncols) check_column(g, c, 0, word);
+}
+
+// Batched: job j owns the columns [col_base[j], col_base[j + 1]).
+template <class T>
+__global__ void check_input_batch_kernel(const InputCheck<T>* jobs, const int64_t* col_base, int njobs, int* word) {
+  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= col_base[njobs]) return;
+  const int j = find_job(col_base, njobs, c);
+  check_column(jobs[j], c - col_base[j], j, word);
+}
+
+// Export of the pass-1 intermediate (the reference's CompressedVolumeWithRadii,
+// src/vor3d/CompressedVolumeWithRadii.h:10-38: per column the z-disjoint
+// triples (y1, y2, r)) from the slot layout into CSR column order x + nx*y.
+// Phase 0 writes each column's triple count, phase 1 the triples at the
+// scanned offsets, with r = sqrt(R^2 - dy^2) and the endpoints plus `shift`.
+template <class T>
+struct TripleExport {
+  int nx, ny, caps;
+  const T* m_lo;
+  const T* m_hi;
+  const uint16_t* m_dy;
+  const int32_t* m_cnt;
+  double R2, shift;
+  int64_t* len;
+  const int64_t* off;
+  double* y1;
+  double* y2;
+  double* r;
+};
+
+template <class T>
+__global__ void export_triples_kernel(TripleExport<T> g, int phase) {
+  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= int64_t(g.nx) * g.ny) return;
+  const int x = int(c % g.nx), y = int(c / g.nx);
+  const int n = min(g.m_cnt[count_index(g.ny, x, y)], g.caps);
+  if (phase == 0) {
+    g.len[c] = n;
+    return;
+  }
+  const int64_t o = g.off[c];
+  for (int j = 0; j < n; ++j) {
+    const int64_t s = slot_index(g.ny, g.caps, x, y, j);
+    const double dy = double(g.m_dy[s]);
+    g.y1[o + j] = double(g.m_lo[s]) + g.shift;
+    g.y2[o + j] = double(g.m_hi[s]) + g.shift;
+    g.r[o + j] = sqrt(fmax(g.R2 - dy * dy, 0.0));
+  }
 }
 
 // Complement on a grid grown by a ring of full columns
@@ -936,6 +1040,10 @@ struct Pass2GArgs {
   const T* s_lo;        // block summaries from pass 1 (null: none, no block is skipped)
   const T* s_hi;
   const T* c_hi;        // per column: hi of its last triple (with the summaries)
+  // CSR index of the column at (line x, position y) of this logical grid:
+  // x * sx + y * sy (sx = 1, sy = nx sweeps along y, the reference's axis;
+  // sx = ny, sy = 1 with nx, ny swapped sweeps along x)
+  int64_t sx, sy;
 };
 
 
@@ -1250,7 +1358,7 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
     return;
   }
   if (!active) return;
-  const int64_t col = int64_t(x) + int64_t(g.nx) * y;
+  const int64_t col = int64_t(x) * g.sx + int64_t(y) * g.sy;
   const int64_t ob = g.o_base + tile * g.arena + int64_t(L) * 2 * ocap;
   for (int i = 0; i < n; ++i) {
     g.o_z[ob + 2 * i] = T(olo[i * kWave]);
@@ -1468,7 +1576,7 @@ __global__ __launch_bounds__(kWave) void pass1_paint_kernel(Pass1Args<T> g) {
   const int y0 = ci * g.chunk;
   const int y1 = min(g.ny, y0 + g.chunk);
   if (y0 >= g.ny) return;
-  auto column = [&](int y) { return int64_t(x) + int64_t(g.nx) * y; };
+  auto column = [&](int y) { return int64_t(x) * g.sx + int64_t(y) * g.sy; };
   // descriptor and first batch of a column, fetched one step ahead
   auto fetch = [&](int y, int64_t& zo, int64_t& np, T& a, T& b) {
     const int64_t col = column(y);
@@ -1589,6 +1697,10 @@ struct Pass1ZArgs {
   T* s_lo;
   T* s_hi;
   T* c_hi;  // per column, at count_index: hi of its last triple (when it has one)
+  // CSR index of the column at (line x, position y) of this logical grid:
+  // x * sx + y * sy (sx = 1, sy = nx sweeps along y, the reference's axis;
+  // sx = ny, sy = 1 with nx, ny swapped sweeps along x)
+  int64_t sx, sy;
 };
 
 constexpr int kZTile = kWave;    // output columns per wavefront
@@ -1603,6 +1715,11 @@ __host__ __device__ constexpr bool zsweep_offsets_fit(int caps, int ny, int tsiz
 }
 constexpr int kMaxSlabs = 64;    // z-slabs per window
 
+// Window columns for the lanes' event masks (walking lanes, R <= kWalkMaxR):
+// bit i of colmask[w] marks event i of the current chunk as one of window
+// column w's.
+constexpr int kColMaskMax = kWave + 2 * kWalkMaxR;
+
 template <class T, int ECAP, int NB, int WMAX>
 struct Pass1ZSmem {
   T ez[ECAP];
@@ -1610,6 +1727,7 @@ struct Pass1ZSmem {
   static_assert((WMAX << 2) < 65536, "window columns fit the event code");
   int bin[NB + 1];
   int slab_end[kMaxSlabs + 1];
+  uint64_t colmask[kColMaskMax];
 };
 
 // Visits the merged segments of one input column (touching or overlapping
@@ -1666,15 +1784,14 @@ struct ZState {
   // loop-carried flags as 0/1 integers: in vector registers they need no
   // exec-mask merges at every join of the walk
   int point_open = 0, ok = 1;
-  // the wavefront's first slot and summary in each array (uniform), and this
-  // lane's slot and summary offsets from them for triple 0
+  // the wavefront's first slot in each array (uniform) and this lane's slot
+  // offset from it for triple 0
   T* lo_w = nullptr;
   T* hi_w = nullptr;
   uint16_t* dy_w = nullptr;
-  T* slo_w = nullptr;
-  T* shi_w = nullptr;
-  uint32_t slot0 = 0, sidx0 = 0;
+  uint32_t slot0 = 0;
   uint32_t ny = 0;
+  T last_hi = T(0);  // hi of the lane's last triple
   // the lane's pending group of points of equal z (their bits stay set until
   // the lane's next event that is not a point at the same z)
   int pend = 0;
@@ -1734,16 +1851,13 @@ struct ZState {
     const bool put = changed && dcur <= R && (z0 < z || point_open != 0) && active;
     if (put && count < g.caps) {
       const uint32_t j = uint32_t(count);
-      const uint32_t blk = j / kSlotBlock;
-      const uint32_t s = slot0 + blk * (ny * kSlotBlock) + j % kSlotBlock;
-      const uint32_t q = sidx0 + blk * ny;
+      const uint32_t s = slot0 + (j / kSlotBlock) * (ny * kSlotBlock) + j % kSlotBlock;
       store_at(lo_w, s, z0);
       store_at(hi_w, s, z);
       store_at(dy_w, s, uint16_t(dcur));
-      if (j % kSlotBlock == 0) store_at(slo_w, q, z0);
-      store_at(shi_w, q, z);
     }
     ok = (put && count >= g.caps) ? 0 : ok;
+    last_hi = put ? z : last_hi;
     count += put ? 1 : 0;
     const bool open = changed && dist <= R;
     z0 = open ? z : z0;
@@ -1783,6 +1897,41 @@ struct ZState {
   }
 };
 
+// Bit i of the result: event i of the chunk (held by lane i, window column
+// `col`) lies within this lane's reach, col - L in [0, 2R].  colmask[w]
+// collects the chunk's events of window column w (one LDS atomic per event);
+// the lane's mask is the OR of colmask over [L, L + 2R], built by doubling in
+// registers: after the step with shift s every entry w covers [w, w + 2s).
+// Entries w = L, L + 64, L + 128 live in lane L (wn <= kColMaskMax).
+__device__ __forceinline__ uint64_t lane_event_mask(uint64_t* colmask, int col, bool has, int wn, int R) {
+  const int L = lane_id();
+  for (int w = L; w < wn; w += kWave) colmask[w] = 0;
+  wave_sync();
+  if (has) atomicOr(reinterpret_cast<unsigned long long*>(&colmask[col]), 1ull << L);
+  wave_sync();
+  const bool three = wn > 2 * kWave;  // uniform
+  uint64_t e0 = colmask[L];
+  uint64_t e1 = L + kWave < wn ? colmask[L + kWave] : 0;
+  uint64_t e2 = three && L + 2 * kWave < wn ? colmask[L + 2 * kWave] : 0;
+  const int W = 2 * R + 1;
+  int span = 1;
+  for (; 2 * span <= W; span *= 2) {
+    const int src = (L + span) & (kWave - 1);
+    const bool wrap = L + span >= kWave;
+    const uint64_t a0 = __shfl(e0, src), a1 = __shfl(e1, src);
+    uint64_t a2 = 0;
+    if (three) a2 = __shfl(e2, src);
+    e0 |= wrap ? a1 : a0;
+    e1 |= wrap ? a2 : a1;
+    e2 |= wrap ? 0 : a2;
+  }
+  // [L, L + W) = [L, L + span) | [L + W - span, L + W), with W - span < span <= 64
+  const int off = W - span;
+  const int src = (L + off) & (kWave - 1);
+  const uint64_t b0 = __shfl(e0, src), b1 = __shfl(e1, src);
+  return e0 | (L + off >= kWave ? b1 : b0);
+}
+
 template <class T, int ECAP, int NB, int WMAX, int NW>
 __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, ECAP, NB, WMAX>& sm) {
   const int L = lane_id();
@@ -1795,7 +1944,7 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
     const int yy = wlo + w;
     s = n = 0;
     if (yy < 0 || yy >= g.ny) return;
-    const int64_t c = int64_t(x) + int64_t(g.nx) * yy;
+    const int64_t c = int64_t(x) * g.sx + int64_t(yy) * g.sy;
     s = g.in_off[c];
     n = g.in_off[c + 1] - s;
   };
@@ -1882,10 +2031,7 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
   st.lo_w = g.m_lo + wslot;
   st.hi_w = g.m_hi + wslot;
   st.dy_w = g.m_dy + wslot;
-  st.slo_w = g.s_lo + wsum;
-  st.shi_w = g.s_hi + wsum;
   st.slot0 = uint32_t(L * kSlotBlock);
-  st.sidx0 = uint32_t(L);
   st.ny = uint32_t(g.ny);
   int b_lo = 0;
   for (int sl = 0; sl < nslab; ++sl) {
@@ -1973,10 +2119,7 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
       bool walk = false;
       uint64_t mine = 0;
       if (R <= kWalkMaxR) {
-        for (int i = 0; i < cnt; ++i) {
-          const int col = __builtin_amdgcn_readlane(vp, i) >> 2;
-          mine |= unsigned(col - L) <= span ? (uint64_t(1) << i) : 0;
-        }
+        mine = lane_event_mask(sm.colmask, vp >> 2, L < cnt, wn, R);
         walk = 4 * wave_max(popc(mine)) < 3 * cnt;
       }
       if (walk) {
@@ -2002,10 +2145,23 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
     b_lo = b_hi;
   }
   if (ballot(active && st.ok == 0) != 0) raise_status(g.status, kStatusArenaOverflow);
+  const int kept = min(st.count, g.caps);
   if (active) g.m_cnt[count_index(g.ny, x, y0 + L)] = st.count;
-  if (active && st.count > 0) {  // the lane's own last summary, read back
-    const uint32_t last = uint32_t(min(st.count, g.caps) - 1);
-    g.c_hi[count_index(g.ny, x, y0 + L)] = st.shi_w[st.sidx0 + (last / kSlotBlock) * st.ny];
+  if (active && st.count > 0) g.c_hi[count_index(g.ny, x, y0 + L)] = st.last_hi;
+  // Block summaries (lo of a block's first triple, hi of its last), written
+  // block by block for all lanes at once so that each store is one
+  // contiguous run; the values are read back from this lane's own blocks.
+  const int nblk = (kept + kSlotBlock - 1) / kSlotBlock;
+  const int nblk_max = wave_max(nblk);
+  T* const slo_w = g.s_lo + wsum;
+  T* const shi_w = g.s_hi + wsum;
+  for (int b = 0; b < nblk_max; ++b) {
+    if (b < nblk) {
+      const uint32_t s = st.slot0 + uint32_t(b) * (st.ny * kSlotBlock);
+      const int last = min(kept - b * kSlotBlock, kSlotBlock) - 1;
+      store_at(slo_w, uint32_t(L) + uint32_t(b) * st.ny, st.lo_w[s]);
+      store_at(shi_w, uint32_t(L) + uint32_t(b) * st.ny, st.hi_w[s + uint32_t(last)]);
+    }
   }
 }
 

@@ -55,6 +55,12 @@ enum : int {
   kStatusListOverflow = kStatusP1List | kStatusP1Union | kStatusP2Interiors | kStatusP2Disks | kStatusP2Slice,
 };
 
+// Input-check bits (a word of their own, written by check_input_kernel).
+enum : int {
+  kInputMalformed = 1,   // decreasing offsets, odd column, unsorted or non-finite endpoint
+  kInputOutOfRange = 2,  // an endpoint outside the range the operation needs
+};
+
 __device__ __forceinline__ void raise_status(int* status, int bits) {
   if (lane_id() == 0) atomicOr(status, bits);
 }

@@ -28,17 +28,50 @@ _DILATIONS = {"noop": 0, "dilation": 1, "erosion": 1, "closing": 2, "opening": 2
 class Resident:
     nx: int
     ny: int
-    off: _native.DeviceArray
-    z: _native.DeviceArray
+    off: object          # device buffer (DeviceArray or a pooled DeviceBuffer)
+    z: object
     z_lo: float          # device z-range (shifted for FP32 storage)
     z_hi: float
     shift: float         # FP32 storage: device z = true z - shift
     spacing: float
     endpoints: int
+    pooled: bool = False  # buffers taken from the server's pool (given back on drop)
 
     @property
     def ncols(self):
         return self.nx * self.ny
+
+
+class DevicePool:
+    """Device buffers kept for reuse.  A result takes a free buffer of the
+    smallest power-of-two size class that holds it (allocating one only when
+    none is free) and a dropped result gives its buffers back, so a steady
+    stream of requests allocates and frees nothing once the pool is warm."""
+
+    def __init__(self):
+        self.free = {}      # size class (bytes) -> [DeviceBuffer]
+        self.allocated = 0  # bytes held by the pool, in use or free
+
+    @staticmethod
+    def size_class(nbytes):
+        return max(256, 1 << max(0, int(nbytes) - 1).bit_length())
+
+    def take(self, nbytes):
+        c = self.size_class(nbytes)
+        lst = self.free.get(c)
+        if lst:
+            return lst.pop()
+        self.allocated += c
+        return _native.DeviceBuffer(c)
+
+    def give(self, buf):
+        self.free.setdefault(buf.nbytes, []).append(buf)
+
+    def close(self):
+        for lst in self.free.values():
+            for b in lst:
+                b.free()
+        self.free.clear()
 
 
 @dataclasses.dataclass
@@ -53,16 +86,21 @@ class Request:
     result: int = -1     # handle of the resident result
 
 
-def footprint_bytes(ncols, endpoints, op, storage_bytes=4, caps=96, ocap=16):
-    """Estimated device bytes a job occupies inside a batch: the pass-1 slots
-    (lo, hi and a 16-bit line offset per slot), the pass-2 tile arenas, the
-    per-column counters and offsets, and the intermediate volumes of its
-    stages (complements grow the grid by a ring)."""
+def footprint_bytes(ncols, endpoints, op, storage_bytes=4, caps=96, ocap=24):
+    """Estimated device bytes a job occupies inside a batch, per column: the
+    pass-1 slots (lo, hi and a 16-bit line offset per slot), their block
+    summaries (lo and hi per 8 slots) and last hi, the triple count, the
+    gather's interval lists (2 * ocap doubles whatever the storage), its
+    output arena (2 * ocap endpoints), the start, length and offset words;
+    plus the intermediate volumes of the job's stages (complements grow the
+    grid by a ring).  ocap is the largest list capacity a batch uses (24 when
+    it holds a radius below 6, 16 otherwise)."""
     if op not in _DILATIONS:
         raise ValueError(f"unknown operation {op!r}")
     if _DILATIONS[op] == 0:
         return 0
-    per_col = caps * (2 * storage_bytes + 2) + 2 * ocap * storage_bytes + 4 + 8 + 8 + 8
+    per_col = (caps * (2 * storage_bytes + 2) + 2 * (caps // 8) * storage_bytes + storage_bytes + 4
+               + 2 * ocap * 8 + 2 * ocap * storage_bytes + 8 + 8 + 8)
     stages = 4 if op in ("closing", "opening") else (3 if op == "erosion" else 1)
     volume = (ncols + 1) * 8 + (endpoints + 2 * ncols) * storage_bytes
     return ncols * per_col + stages * volume
@@ -78,6 +116,7 @@ class MorphoServer:
         self.ctx = ctx if ctx is not None else _native.Context(device)
         self.vols: dict[int, Resident] = {}
         self.queue: collections.deque[Request] = collections.deque()
+        self.pool = DevicePool()
         self._next = 0
         self._rid = 0
         self.batches = []    # (size, seconds) of every executed batch
@@ -108,8 +147,12 @@ class MorphoServer:
 
     def drop(self, handle):
         r = self.vols.pop(handle)
-        r.off.free()
-        r.z.free()
+        if r.pooled:
+            self.pool.give(r.off)
+            self.pool.give(r.z)
+        else:
+            r.off.free()
+            r.z.free()
 
     # -- requests ----------------------------------------------------------
     def submit(self, handle, op, radius, arrival=None):
@@ -153,18 +196,19 @@ class MorphoServer:
         for req in batch:
             req.start = t0
         outs, _ = self.ctx.morph_batch_device(self.precision, jobs)
-        dtype = np.float32 if self.precision == "fp32" else np.float64
+        esize = 4 if self.precision == "fp32" else 8
+        # results leave the context's workspace (reused by the next batch) for
+        # pooled buffers: no allocation once the pool is warm, no host data
+        copies = []
         for req, (v, cnt) in zip(batch, outs):
             src = self.vols[req.handle]
             ncols = v.nx * v.ny
-            off = _native.DeviceArray(np.zeros(ncols + 1, np.int64))
-            z = _native.DeviceArray(np.zeros(max(cnt, 1), dtype))
-            _native.copy_device(off.address, v.offsets, (ncols + 1) * 8)
-            if cnt:
-                _native.copy_device(z.address, v.z, cnt * np.dtype(dtype).itemsize)
+            off = self.pool.take((ncols + 1) * 8)
+            z = self.pool.take(max(cnt, 1) * esize)
+            copies += [(off.address, v.offsets, (ncols + 1) * 8), (z.address, v.z, cnt * esize)]
             req.result = self._register(Resident(v.nx, v.ny, off, z, src.z_lo, src.z_hi, src.shift, src.spacing,
-                                                 int(cnt)))
-        _native.synchronize()
+                                                 int(cnt), pooled=True))
+        self.ctx.copy_device(copies)  # one synchronisation for the batch
         t1 = time.perf_counter()
         for req in batch:
             req.done = t1
@@ -177,6 +221,7 @@ class MorphoServer:
     def close(self):
         for h in list(self.vols):
             self.drop(h)
+        self.pool.close()
         self.ctx.close()
 
 

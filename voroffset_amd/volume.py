@@ -104,6 +104,20 @@ class CompressedVolume:
                 total += s * (col[k + 1] - col[k])
         return total
 
+    def crop(self, x0, y0, w, h):
+        """The w x h sub-grid starting at column (x0, y0), with this volume's
+        geometry (its z-range, which erosion depends on, is unchanged)."""
+        if not (0 <= x0 and 0 <= y0 and w >= 0 and h >= 0 and x0 + w <= self.nx and y0 + h <= self.ny):
+            raise ValueError("crop outside the grid")
+        xs = np.arange(x0, x0 + w)
+        cols = (xs[None, :] + self.nx * np.arange(y0, y0 + h)[:, None]).ravel()
+        lens = self.off[cols + 1] - self.off[cols]
+        off = np.zeros(cols.size + 1, np.int64)
+        np.cumsum(lens, out=off[1:])
+        idx = np.repeat(self.off[cols] - off[:-1], lens) + np.arange(off[-1])
+        return CompressedVolume(w, h, off, self.z[idx].copy(), self.origin, self.extent, self.spacing,
+                                self.padding, dict(self.meta))
+
     # -- text format (CompressedVolume.cpp:116-152) -------------------------
     def save(self, path):
         with open(path, "w") as f:
