@@ -125,6 +125,23 @@ typedef struct {
 int vo_morph_batch_device(vo_context* ctx, int precision, int n, const vo_job* jobs, vo_volume* out,
                           int64_t* out_count, double* time_1_ms, double* time_2_ms);
 
+/* vo_morph_device without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * operations, kernels and results as vo_morph_device; the workspace is one
+ * 64 MiB array per GPU (a static device array, held until the process ends),
+ * and a volume whose workspace would not fit in it is rejected with
+ * VO_ERR_CAPACITY (use vo_morph_device for it).  The caller passes the output
+ * buffers: out_offsets (nx*ny + 1 int64, the result's offsets from 0) and
+ * out_z (out_capacity >= 1 endpoints of the storage type); when out_z is too
+ * small the call returns VO_ERR_CAPACITY with *out_count the capacity
+ * needed.  All calls on one GPU share its scratch, which keeps the last
+ * call's contents: calls that overlap on one GPU (other contexts, other
+ * threads) race and must be serialised by the caller. */
+int vo_morph_device_noalloc(vo_context* ctx, int op, int precision, double radius, const vo_volume* in,
+                            int64_t* out_offsets, void* out_z, int64_t out_capacity, int64_t* out_count,
+                            double* time_1_ms, double* time_2_ms);
+
 /* n device-to-device copies dst[i] <- src[i] (bytes[i] each) on the
  * context's stream, then one synchronisation: how a caller keeps results of
  * vo_morph_batch_device / vo_morph_device beyond the next call, into buffers

@@ -23,7 +23,7 @@ EXPORTS = ("vo_create", "vo_destroy", "vo_last_error", "vo_morph", "vo_result_sh
            "vo_result_free", "vo_morph_device", "vo_last_stats", "vo_rasterize", "vo_morph_batch_device",
            "vo_last_batch_alone", "vo_measure_device", "vo_xor_device",
            "vo_dexel_mesh", "vo_mesh_shape", "vo_mesh_copy", "vo_mesh_free", "vo_autotune", "vo_tuning_save",
-           "vo_tuning_load", "vo_tuning_get", "vo_pass1", "vo_triples_shape", "vo_triples_copy", "vo_triples_free", "vo_copy_device")
+           "vo_tuning_load", "vo_tuning_get", "vo_pass1", "vo_triples_shape", "vo_triples_copy", "vo_triples_free", "vo_copy_device", "vo_morph_device_noalloc")
 
 
 class VoError(RuntimeError):
@@ -108,6 +108,9 @@ def lib():
         L.vo_triples_copy.restype = i32
         L.vo_triples_free.argtypes = [vp]
         L.vo_triples_free.restype = None
+        L.vo_morph_device_noalloc.argtypes = [vp, i32, i32, dbl, ctypes.POINTER(VoVolume), vp, vp, i64,
+                                              ctypes.POINTER(i64), ctypes.POINTER(dbl), ctypes.POINTER(dbl)]
+        L.vo_morph_device_noalloc.restype = i32
         L.vo_copy_device.argtypes = [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(i64)]
         L.vo_copy_device.restype = i32
         L.vo_last_batch_alone.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
@@ -210,6 +213,23 @@ class Context:
         if rc != OK:
             self._raise(rc)
         return [(outs[i], cnt[i]) for i in range(n)], (t1.value, t2.value)
+
+    def morph_device_noalloc(self, op, precision, radius, nx, ny, off_ptr, z_ptr, z_lo, z_hi, out_off_ptr,
+                             out_z_ptr, capacity):
+        """vo_morph_device_noalloc: the result goes to the caller's device
+        buffers; returns (endpoint count, (time_1, time_2)).  On a too small
+        out_z the VoError carries code ERR_CAPACITY and `.needed`."""
+        vin = VoVolume(nx, ny, off_ptr, z_ptr, z_lo, z_hi)
+        cnt = ctypes.c_int64(0)
+        t1, t2 = ctypes.c_double(0), ctypes.c_double(0)
+        rc = lib().vo_morph_device_noalloc(self._h, OP_CODES[op], PRECISIONS[precision], float(radius),
+                                           ctypes.byref(vin), out_off_ptr, out_z_ptr, int(capacity),
+                                           ctypes.byref(cnt), ctypes.byref(t1), ctypes.byref(t2))
+        if rc != OK:
+            err = VoError(rc, lib().vo_last_error(self._h).decode())
+            err.needed = cnt.value
+            raise err
+        return cnt.value, (t1.value, t2.value)
 
     def copy_device(self, copies):
         """Device-to-device copies [(dst, src, nbytes)] on this context's

@@ -20,6 +20,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -108,6 +109,30 @@ struct DevVol {
 
 }  // namespace
 
+// The no-alloc entry point's workspace: one array per GPU, held until the
+// process ends.  Host code never uses its name as an address (that is a
+// host-side stand-in): vo_create looks up each device's copy with
+// hipGetSymbolAddress once, with that device current, and passes that
+// address on.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// Bump allocation over a fixed device buffer: the workspace of the entry
+// point that allocates nothing (vo_morph_device_noalloc).  Each slot keeps
+// the region it was given for the call; `fixed` slots are the caller's
+// output buffers, never moved.
+struct ScratchArena {
+  bool on = false;
+  char* base = nullptr;
+  size_t cap = 0, used = 0;
+  size_t short_by = 0;  // bytes an output buffer needed when it was too small
+  Buffer slot[kSLOTS];
+  bool fixed[kSLOTS] = {};
+};
+
 struct vo_context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -125,6 +150,8 @@ struct vo_context {
   Tuning tuning;             // per radius bucket (vo_autotune, vo_tuning_load)
   int pass1_form = 0;        // VO_PASS1: 0 = z-sweep (default), 1 = "paint", 2 = "cull"
   int axis_forced = -1;      // VO_AXIS: -1 = chosen per volume, 0 = pass 1 along y, 1 = along x
+  void* scratch = nullptr;   // this device's no-alloc scratch (device address, from hipGetSymbolAddress)
+  ScratchArena arena;        // on during vo_morph_device_noalloc
   int last_fallback_tiles = 0;
   std::vector<void*> batch_extra;  // outputs of batched jobs redone alone (freed by the next batch)
   // host copies of the input-check tables (every call ends synchronised, so
@@ -192,6 +219,24 @@ void sync(vo_context* ctx) {
 }
 
 void* ws(vo_context* ctx, int slot, size_t bytes) {
+  if (ctx->arena.on) {  // the no-alloc entry point: regions of the scratch
+    ScratchArena& a = ctx->arena;
+    Buffer& b = a.slot[slot];
+    if (bytes == 0) bytes = 16;
+    if (b.bytes >= bytes) return b.ptr;
+    if (a.fixed[slot]) a.short_by = bytes;
+    if (a.fixed[slot])
+      fail(ctx, VO_ERR_CAPACITY, "no-alloc: the output buffer holds " + std::to_string(b.bytes) + " bytes, " +
+                                     std::to_string(bytes) + " are needed");
+    const size_t at = (a.used + 255) & ~size_t(255);
+    if (at + bytes > a.cap)
+      fail(ctx, VO_ERR_CAPACITY, "no-alloc: the " + std::to_string(a.cap >> 20) +
+                                     " MiB scratch is too small for this volume (use vo_morph_device)");
+    b.ptr = a.base + at;
+    b.bytes = bytes;
+    a.used = at + bytes;
+    return b.ptr;
+  }
   Buffer& b = ctx->buf[slot];
   if (bytes == 0) bytes = 16;
   if (b.bytes < bytes) {
@@ -207,8 +252,8 @@ void* ws(vo_context* ctx, int slot, size_t bytes) {
   return b.ptr;
 }
 
-// The buffer ws() last gave for `slot`.
-void* slot_ptr(vo_context* ctx, int slot) { return ctx->buf[slot].ptr; }
+// The buffer ws() last gave for `slot` in this mode.
+void* slot_ptr(vo_context* ctx, int slot) { return ctx->arena.on ? ctx->arena.slot[slot].ptr : ctx->buf[slot].ptr; }
 
 unsigned blocks_for(int64_t n, int threads) { return unsigned((n + threads - 1) / threads); }
 
@@ -1426,6 +1471,40 @@ void pass1_host(vo_context* ctx, double R, const vo_volume* in, vo_triples* res)
   sync(ctx);
 }
 
+// vo_morph_device with every buffer in this device's scratch and the result
+// in the caller's buffers: the operation runs as in vo_morph_device (same
+// kernels, same results), its workspace handed out by ScratchArena.
+template <class T>
+void morph_noalloc(vo_context* ctx, int op, double R, const vo_volume* in, int64_t* out_offsets, void* out_z,
+                   int64_t out_capacity, int64_t* out_count, double* t1, double* t2) {
+  validate_shape(ctx, in);
+  if (op < VO_OP_DILATION || op > VO_OP_OPENING) fail(ctx, VO_ERR_ARGUMENT, "unknown operation");
+  if (!out_offsets || !out_z || !out_count || out_capacity < 1)
+    fail(ctx, VO_ERR_ARGUMENT, "no-alloc: null output or capacity below 1");
+  if (!ctx->scratch) fail(ctx, VO_ERR_DEVICE, "no-alloc: no scratch on this device");
+  const int64_t n = int64_t(in->nx) * in->ny;
+  ScratchArena& a = ctx->arena;
+  a = ScratchArena();
+  a.on = true;
+  a.base = static_cast<char*>(ctx->scratch);
+  a.cap = kScratchBytes;
+  a.slot[kV3_OFF] = Buffer{out_offsets, size_t(n + 1) * sizeof(int64_t)};
+  a.fixed[kV3_OFF] = true;
+  a.slot[kV3_Z] = Buffer{out_z, size_t(out_capacity) * sizeof(T)};
+  a.fixed[kV3_Z] = true;
+  DevVol v;
+  v.nx = in->nx;
+  v.ny = in->ny;
+  v.off = in->offsets;
+  v.z = in->z;
+  enqueue_input_check<T>(ctx, {input_check_of<T>(op, R, *in)}, {n}, {0});
+  DevVol r = run_op<T>(ctx, op, v, R, in->z_lo, in->z_hi, t1, t2);
+  if (r.off != out_offsets || (r.count > 0 && r.z != out_z))
+    fail(ctx, VO_ERR_DEVICE, "no-alloc: the result is not in the caller's buffers");
+  sync(ctx);
+  *out_count = r.count;
+}
+
 // Why this thread's last vo_create failed (vo_last_error(NULL)).
 thread_local std::string g_create_error;
 
@@ -1452,6 +1531,15 @@ int vo_create(vo_context** out, int device) {
     for (auto& e : ctx->ev) check(ctx, hipEventCreate(&e), "event");
     check(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->host_words), 8 * sizeof(int64_t), hipHostMallocDefault),
           "pinned");
+    if (device < kMaxDevices) {  // this device's copy of the no-alloc scratch (device current: guarded)
+      std::lock_guard<std::mutex> lock(g_scratch_mu);
+      if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        check(ctx, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[device] = p;
+      }
+      ctx->scratch = g_scratch_ptr[device];
+    }
   });
   if (rc != VO_OK) {
     g_create_error = ctx->error;
@@ -1549,6 +1637,26 @@ int vo_triples_copy(const vo_triples* t, int64_t* offsets, double* y1, double* y
 }
 
 void vo_triples_free(vo_triples* t) { delete t; }
+
+int vo_morph_device_noalloc(vo_context* ctx, int op, int precision, double radius, const vo_volume* in,
+                            int64_t* out_offsets, void* out_z, int64_t out_capacity, int64_t* out_count,
+                            double* time_1_ms, double* time_2_ms) {
+  const int rc = guarded(ctx, [&] {
+    if (precision == VO_FP32)
+      morph_noalloc<float>(ctx, op, radius, in, out_offsets, out_z, out_capacity, out_count, time_1_ms, time_2_ms);
+    else if (precision == VO_FP64)
+      morph_noalloc<double>(ctx, op, radius, in, out_offsets, out_z, out_capacity, out_count, time_1_ms, time_2_ms);
+    else
+      fail(ctx, VO_ERR_ARGUMENT, "unknown precision");
+  });
+  if (ctx) {
+    // an output buffer too small: *out_count is the capacity needed
+    if (rc == VO_ERR_CAPACITY && ctx->arena.short_by && out_count)
+      *out_count = int64_t(ctx->arena.short_by / (precision == VO_FP32 ? sizeof(float) : sizeof(double)));
+    ctx->arena.on = false;
+  }
+  return rc;
+}
 
 int vo_copy_device(vo_context* ctx, int n, void* const* dst, const void* const* src, const int64_t* bytes) {
   return guarded(ctx, [&] {
