@@ -982,6 +982,21 @@ struct FinalJob {
   int64_t out_base;
 };
 
+// Endpoint count of every job's volume, off[ncols] - off[0].
+__global__ void job_counts_kernel(const FinalJob* jobs, int njobs, int64_t* count) {
+  const int j = int(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= njobs) return;
+  const FinalJob g = jobs[j];
+  count[j] = g.ncols > 0 ? g.off[g.ncols] - g.off[0] : 0;
+}
+
+// n 32-bit words set to v: what the batch program enqueues where a memset
+// would go, so that the program can be captured in a HIP graph.
+__global__ void fill_kernel(uint32_t* p, int64_t n, uint32_t v) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 __global__ void finalize_offsets_kernel(const FinalJob* jobs, const int64_t* entry_base, int njobs, int64_t* rel,
                                         int64_t* first, int64_t* count) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1718,7 +1733,7 @@ constexpr int kMaxSlabs = 64;    // z-slabs per window
 // Window columns for the lanes' event masks (walking lanes, R <= kWalkMaxR):
 // bit i of colmask[w] marks event i of the current chunk as one of window
 // column w's.
-constexpr int kColMaskMax = kWave + 2 * kWalkMaxR;
+constexpr int kColMaskMax = kWave + 2 * kWalkMaxR;  // (also >= kWave + 1: the cell walk's difference array)
 
 template <class T, int ECAP, int NB, int WMAX>
 struct Pass1ZSmem {
@@ -1897,6 +1912,157 @@ struct ZState {
   }
 };
 
+// Pass 1 as a walk over Voronoi cells (windows without zero-length
+// segments).  At height z the covered window columns split the line into
+// cells: output column L (window position L + R) is in the cell of its
+// nearest covered column, and its distance d_L is what the triples record.
+// An event at column c changes d only inside c's cell: for a start, the
+// positions strictly nearer to c than to the covered columns p < c and
+// q > c around it (d becomes |c - pos|); for an end, the positions whose
+// nearest was c alone (d becomes the distance to p or q, or "none" beyond
+// R).  So each event touches a contiguous run of a few output columns, not
+// all 2R + 1 within reach, and the walk costs about one step per triple.
+//
+// Events are read 64 at a time in (z, kind) order.  Lane i holds event i:
+// an inclusive XOR scan of one-hot column bits over the lanes, on top of the
+// covered mask before the chunk, gives the mask before each event; p and q
+// are bit scans in it; the event's run of output columns [a, b] becomes
+// bit i of the lanes in the run through an XOR difference array (the bits
+// are distinct, so OR = XOR) and a prefix XOR.  Then every lane walks its
+// own events in order, as the lane walk does.
+constexpr int kCellMaxR = 63;
+
+template <class T, int MW>
+struct CellWalk {
+  // covered window columns before the current chunk (uniform)
+  uint64_t carry[MW];
+  int R;
+  bool active;
+  T* lo_w;
+  T* hi_w;
+  uint16_t* dy_w;
+  uint32_t slot0, ny;
+  // the lane's state: its open piece starts at z0 with distance dcur
+  // (R + 1: no piece)
+  T z0 = T(0);
+  T last_hi = T(0);
+  int dcur, count = 0, ok = 1;
+
+  __device__ __forceinline__ CellWalk(int r, bool act, T* lo, T* hi, uint16_t* dy, uint32_t s0, uint32_t n)
+      : R(r), active(act), lo_w(lo), hi_w(hi), dy_w(dy), slot0(s0), ny(n), dcur(r + 1) {
+#pragma unroll
+    for (int t = 0; t < MW; ++t) carry[t] = 0;
+  }
+
+  static constexpr int kNone = 0xff;
+
+  // sweep the E events (ez, ep) of one z-slab; `diff` is LDS scratch of
+  // kWave + 1 words
+  template <class A>
+  __device__ void sweep(const T* ez, const uint16_t* ep, int E, uint64_t* diff, const A& g) {
+    const int L = lane_id();
+    for (int base = 0; base < E; base += kWave) {
+      const int cnt = min(kWave, E - base);
+      const bool has = L < cnt;
+      const T vz = has ? ez[base + L] : T(0);
+      const int code = has ? ep[base + L] : 0;
+      const int c = code >> 2;
+      const bool start = (code & 3) == 0;
+      // one-hot column bits, inclusive XOR scan over the lanes
+      uint64_t x[MW], own[MW];
+#pragma unroll
+      for (int t = 0; t < MW; ++t) {
+        own[t] = (has && (c >> 6) == t) ? (uint64_t(1) << (c & 63)) : 0;
+        x[t] = own[t];
+      }
+#pragma unroll
+      for (int d = 1; d < kWave; d <<= 1) {
+#pragma unroll
+        for (int t = 0; t < MW; ++t) {
+          const uint64_t o = __shfl_up(x[t], d);
+          x[t] ^= L >= d ? o : 0;
+        }
+      }
+      // covered columns other than c, just before (start) or after (end)
+      // the event: the mask before it without c
+      uint64_t m[MW];
+#pragma unroll
+      for (int t = 0; t < MW; ++t) m[t] = (carry[t] ^ x[t] ^ own[t]) & ~own[t];
+#pragma unroll
+      for (int t = 0; t < MW; ++t) carry[t] ^= __shfl(x[t], cnt - 1);
+      // p: highest covered column below c, q: lowest above (kNone: none)
+      int p = -1, q = 1 << 20;
+#pragma unroll
+      for (int t = 0; t < MW; ++t) {
+        const int lo_c = c - 64 * t;  // bits of word t below c: [0, lo_c)
+        const uint64_t below = lo_c <= 0 ? 0 : (lo_c >= 64 ? m[t] : (m[t] & ((uint64_t(1) << lo_c) - 1)));
+        p = below ? 64 * t + 63 - __builtin_clzll(below) : p;
+      }
+#pragma unroll
+      for (int t = MW - 1; t >= 0; --t) {
+        const int hi_c = c + 1 - 64 * t;  // bits of word t above c: [hi_c, 64)
+        const uint64_t above = hi_c >= 64 ? 0 : (hi_c <= 0 ? m[t] : (m[t] & (~uint64_t(0) << hi_c)));
+        q = above ? 64 * t + __builtin_ctzll(above) : q;
+      }
+      // c's cell among the output positions [R, R + 63], within reach of c
+      int a = max(c - R, p >= 0 ? ((p + c) >> 1) + 1 : c - R);
+      int b = min(c + R, q < (1 << 20) ? ((c + q + 1) >> 1) - 1 : c + R);
+      a = max(a, R) - R;
+      b = min(b, R + kWave - 1) - R;
+      // bit L of the run [a, b]: XOR difference array, then a prefix XOR
+      if (L <= kWave) diff[L] = 0;
+      wave_sync();
+      if (has && a <= b) {
+        const unsigned long long bit = 1ull << L;
+        atomicXor(reinterpret_cast<unsigned long long*>(&diff[a]), bit);
+        atomicXor(reinterpret_cast<unsigned long long*>(&diff[b + 1]), bit);
+      }
+      wave_sync();
+      uint64_t mine = diff[L];
+#pragma unroll
+      for (int d = 1; d < kWave; d <<= 1) {
+        const uint64_t o = __shfl_up(mine, d);
+        mine ^= L >= d ? o : 0;
+      }
+      // the event as one word for the lanes that read it: c, p + 1, q + 1
+      // (kNone beyond the window), start flag
+      const int pk = c | ((p >= 0 ? p + 1 : 0) << 8) | ((q < (1 << 20) ? q + 1 : kNone) << 16) |
+                     (start ? 1 << 24 : 0);
+      const int pos = L + R;
+      for (;;) {
+        const bool on = mine != 0;
+        if (ballot(on) == 0) break;
+        const int i = on ? int(__builtin_ctzll(mine)) : 0;
+        mine &= mine - 1;
+        const T z = __shfl(vz, i);
+        const int e = __shfl(pk, i);
+        const int ec = e & 0xff, ep1 = (e >> 8) & 0xff, eq1 = (e >> 16) & 0xff;
+        int dn;
+        if (e >> 24) {
+          dn = abs(ec - pos);
+        } else {
+          const int dp = ep1 ? pos - (ep1 - 1) : 1 << 20;
+          const int dq = eq1 != kNone ? (eq1 - 1) - pos : 1 << 20;
+          dn = min(min(dp, dq), R + 1);
+        }
+        const bool put = on && dcur <= R && z0 < z && active;
+        if (put && count < g.caps) {
+          const uint32_t j = uint32_t(count);
+          const uint32_t s = slot0 + (j / kSlotBlock) * (ny * kSlotBlock) + j % kSlotBlock;
+          store_at(lo_w, s, z0);
+          store_at(hi_w, s, z);
+          store_at(dy_w, s, uint16_t(dcur));
+        }
+        ok = (put && count >= g.caps) ? 0 : ok;
+        last_hi = put ? z : last_hi;
+        count += put ? 1 : 0;
+        z0 = on ? z : z0;
+        dcur = on ? dn : dcur;
+      }
+    }
+  }
+};
+
 // Bit i of the result: event i of the chunk (held by lane i, window column
 // `col`) lies within this lane's reach, col - L in [0, 2R].  colmask[w]
 // collects the chunk's events of window column w (one LDS atomic per event);
@@ -1973,14 +2139,19 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
   // histogram of the window's endpoints
   for (int b = L; b <= NB; b += kWave) sm.bin[b] = 0;
   wave_sync();
+  int points = 0;
   for (int w = L; w < wn; w += kWave) {
     int64_t s, n;
     col_range(w, s, n);
     column_segments(g.in_z, s, n, [&](T a, T b) {
       atomicAdd(&sm.bin[bin_of(a)], 1);
       if (a < b) atomicAdd(&sm.bin[bin_of(b)], 1);
+      points |= a < b ? 0 : 1;
     });
   }
+  // windows without zero-length segments take the cell walk (R <= 63; the
+  // lane state for R <= 31 has room for 128 window columns)
+  const bool cells = NW <= 2 && ballot(points != 0) == 0 && R <= (NW == 1 ? 31 : kCellMaxR);
   wave_sync();
   // slabs of consecutive bins holding at most ECAP endpoints each: one slab
   // when everything fits (the usual case), else a greedy split
@@ -2033,6 +2204,8 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
   st.dy_w = g.m_dy + wslot;
   st.slot0 = uint32_t(L * kSlotBlock);
   st.ny = uint32_t(g.ny);
+  // covered-mask words of the cell walk: the window's 64 + 2R columns
+  CellWalk<T, NW == 1 ? 2 : 3> cw(R, active, st.lo_w, st.hi_w, st.dy_w, st.slot0, st.ny);
   int b_lo = 0;
   for (int sl = 0; sl < nslab; ++sl) {
     const int b_hi = sm.slab_end[sl];
@@ -2103,8 +2276,17 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
       }
     }
     wave_sync();
-    // Sweep in (z, kind) order.  Events are read 64 at a time into registers;
-    // a lane's events are those of the 2R + 1 window columns within its reach
+    // Sweep in (z, kind) order: the cell walk when it applies, else the lane
+    // walk below.
+    if constexpr (NW <= 2) {
+      if (cells) {
+        cw.sweep(sm.ez, sm.ep, E, sm.colmask, g);
+        b_lo = b_hi;
+        continue;
+      }
+    }
+    // Lane walk: events are read 64 at a time into registers; a lane's
+    // events are those of the 2R + 1 window columns within its reach
     // (k = column - L in [0, 2R]).  For R <= kWalkMaxR each lane marks its
     // events of the chunk and, when no lane has more than 3/4 of the chunk
     // marked, the lanes walk their own marked events in lock step, fetching
@@ -2143,6 +2325,11 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
     // equal z never straddles a slab (slabs end at bin boundaries)
     if (ballot(st.pend != 0) != 0) st.close_points(st.pend != 0, active, g);
     b_lo = b_hi;
+  }
+  if (cells) {  // the cell walk's lane state is the result
+    st.count = cw.count;
+    st.ok = cw.ok;
+    st.last_hi = cw.last_hi;
   }
   if (ballot(active && st.ok == 0) != 0) raise_status(g.status, kStatusArenaOverflow);
   const int kept = min(st.count, g.caps);
