@@ -459,13 +459,24 @@ def main():
         b2 = ncols * ((3 * m * E + 4) + (2 * s_out * E + 4))
         name, b, ms = ("pass2", b2, p2_ms) if p2_ms >= p1_ms else ("pass1", b1, p1_ms)
         achieved = b / (ms * 1e-3) / 1e9
-        traffic = None
+        # the instantiation the timed steps ran (FP32 storage, this radius and
+        # slab capacity); counters measured on another one are not attached
+        events = (tuned or {}).get("events_per_slab", 768)
+        nb = {512: 128, 768: 256, 1024: 256, 1536: 512}.get(events, 256)
+        nw = 1 if int(R) <= 31 else (2 if int(R) <= 63 else 4)
+        inst = (f"pass1_zsweep_kernel<float, {events}, {nb}, 320, {nw}>" if name == "pass1"
+                else f"pass2_gather_kernel<float, {'true' if int(R) <= 31 else 'false'}>")
+        traffic, tnote = None, "no counter profile of this kernel in profiles/"
         tpath = os.path.join(ROOT, "profiles", f"traffic_{name}_1024_r16.json")
         if os.path.exists(tpath):
-            traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+            prof = json.load(open(tpath))
+            if prof.get("kernel") == inst:
+                traffic, tnote = prof.get("hbm_bytes_per_launch"), os.path.relpath(tpath, ROOT)
+            else:
+                tnote = f"{os.path.relpath(tpath, ROOT)} profiles {prof.get('kernel')}, not the timed {inst}"
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": name,
-                    "kernel_ms": round(ms, 4), "algorithmic_bytes": int(b),
+                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": tnote,
+                    "kernel": inst, "kernel_ms": round(ms, 4), "algorithmic_bytes": int(b),
                     "step_algorithmic_GBs": round((b1 + b2) / step_s / 1e9, 2)}
 
     out = {

@@ -113,9 +113,14 @@ int vo_morph_device(vo_context* ctx, int op, int precision, double radius, const
  * names the first failing job.
  * out[i] receives device pointers into the context's workspace (offsets
  * starting at 0), valid until the next call on `ctx`; out_count[i] is its
- * number of endpoints.  time_1 / time_2 sum the batched pass-1 / pass-2
- * kernels.  Volumes whose working lists outgrow the batched kernels are redone
- * one by one with the single-volume fallbacks; results do not depend on it. */
+ * number of endpoints.  The batch runs as one program: output sizes are
+ * bounded before it starts, so the host waits once for the input sizes and
+ * once at its end; a batch identical to the previous one (same jobs,
+ * pointers, sizes) runs from a HIP graph captured on its second call
+ * (VO_BATCH_GRAPH=0 at vo_create disables that).  time_1 is the program's
+ * device time (all primitives), time_2 that of the jobs redone one by one.
+ * Volumes whose working lists outgrow the batched kernels are redone one by
+ * one with the single-volume fallbacks; results do not depend on it. */
 typedef struct {
   int op;         /* vo_op */
   double radius;  /* dexels */

@@ -27,7 +27,7 @@ def mixed_items(seed, n=10, side=(12, 60), r=(1.0, 9.0)):
 def test_batch_equals_single_path_fp64(hip_fp64):
     items = mixed_items(1)
     res, t1, t2 = hip_fp64.apply_batch(items)
-    assert t1 > 0 and t2 > 0
+    assert t1 > 0 and t2 >= 0  # the program's device time; jobs redone alone add to t2
     for (op, vol, r), got in zip(items, res):
         one, _, _ = hip_fp64.apply(op, vol, r)
         assert (got.nx, got.ny) == (vol.nx, vol.ny)
@@ -96,3 +96,41 @@ def test_batch_uneven_specs_fp64(hip_fp64):
     for (op, vol, r), got in zip(items, res):
         bad, err = compare_csr(got.off, got.z, *oracle.morph(vol, op, r, threads=4))
         assert len(bad) == 0 and err <= 1e-9, (op, r, vol.nx, bad[:5], err)
+
+
+def test_batch_program_replays_from_a_graph():
+    """The same batch (same jobs, pointers and sizes) runs eagerly, is
+    captured on its second call and replayed after that: every call's results
+    equal the first's, with other work on the context between the calls."""
+    from voroffset_amd import _native
+    items = mixed_items(5, n=8)
+    ctx = _native.Context(0)
+    keep, jobs = [], []
+    try:
+        for op, vol, r in items:
+            zlo, zhi = vol.z_range()
+            d_off, d_z = _native.DeviceArray(vol.off), _native.DeviceArray(vol.z)
+            keep += [d_off, d_z]
+            jobs.append((op, r, vol.nx, vol.ny, d_off.address, d_z.address, zlo, zhi))
+
+        def results():
+            outs, _ = ctx.morph_batch_device("fp64", jobs)
+            return [(_native.to_host(v.offsets, np.int64, v.nx * v.ny + 1), _native.to_host(v.z, np.float64, c))
+                    for v, c in outs]
+        first = results()
+        for k in range(4):
+            got = results()
+            for (o0, z0), (o1, z1) in zip(first, got):
+                assert np.array_equal(o0, o1) and np.array_equal(z0, z1), k
+            # eager work between the calls
+            op, vol, r = items[k]
+            zlo, zhi = vol.z_range()
+            ctx.morph_device("dilation", "fp64", 2.0, vol.nx, vol.ny, keep[2 * k].address, keep[2 * k + 1].address,
+                             zlo, zhi)
+        for (op, vol, r), (o, z) in zip(items, first):
+            bad, err = compare_csr(o, z, *oracle.morph(vol, op, r))
+            assert len(bad) == 0 and err <= 1e-9, (op, r)
+    finally:
+        for d in keep:
+            d.free()
+        ctx.close()

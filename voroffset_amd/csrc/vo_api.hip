@@ -133,6 +133,13 @@ struct ScratchArena {
   bool fixed[kSLOTS] = {};
 };
 
+// The last batch program run on a context: its key (tables, launch shapes,
+// buffers) and, from its second identical call on, its captured graph.
+struct BatchGraph {
+  std::vector<char> key;
+  hipGraphExec_t exec = nullptr;
+};
+
 struct vo_context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -152,6 +159,10 @@ struct vo_context {
   int axis_forced = -1;      // VO_AXIS: -1 = chosen per volume, 0 = pass 1 along y, 1 = along x
   void* scratch = nullptr;   // this device's no-alloc scratch (device address, from hipGetSymbolAddress)
   ScratchArena arena;        // on during vo_morph_device_noalloc
+  int64_t* batch_host = nullptr;  // pinned: a batch program's counts and statuses
+  size_t batch_host_n = 0;
+  BatchGraph bgraph;              // the last batch program, captured on its second call
+  bool use_graphs = true;         // VO_BATCH_GRAPH=0: never capture
   int last_fallback_tiles = 0;
   std::vector<void*> batch_extra;  // outputs of batched jobs redone alone (freed by the next batch)
   // host copies of the input-check tables (every call ends synchronised, so
@@ -618,7 +629,9 @@ void enqueue_input_check(vo_context* ctx, const std::vector<vo::InputCheck<T>>& 
   if (total > 0) {
     const dim3 grid(blocks_for(total, 256));
     if (K == 1) {
-      hipLaunchKernelGGL(vo::check_input_kernel<T>, grid, dim3(256), 0, ctx->stream, jobs[0], total, word);
+      // four wavefronts of 64 columns per block
+      hipLaunchKernelGGL(vo::check_input_kernel<T>, dim3(blocks_for(total, 256)), dim3(256), 0, ctx->stream, jobs[0],
+                         total, word);
     } else {
       const size_t tb = size_t(K) * sizeof(vo::InputCheck<T>);
       ctx->check_tab.resize(tb);
@@ -740,208 +753,6 @@ DevVol keep_alone(vo_context* ctx, const DevVol& r) {
   return k;
 }
 
-// Dilation of the volumes vol[idx[...]] by R[...]: results replace vol[...]
-// (offsets absolute into one shared endpoint buffer).
-template <class T>
-void dilate_batch(vo_context* ctx, std::vector<DevVol>& vol, const std::vector<double>& R,
-                  const std::vector<int>& idx, int off_slot, int z_slot, double* t1, double* t2) {
-  std::vector<int> jobs, alone;
-  for (int j : idx) {
-    if (!(R[j] >= 0) || !std::isfinite(R[j]) || R[j] > 60000) fail(ctx, VO_ERR_ARGUMENT, "radius must be finite and >= 0");
-    const int rint = int(std::floor(R[j]));
-    if (rint > 127 || int64_t(vol[j].nx) * vol[j].ny == 0 || ctx->force_sweep || ctx->pass1_form != 0 ||
-        !vo::zsweep_offsets_fit(ctx->p1_caps, axis_for(ctx, vol[j].nx, vol[j].ny).ny, int(sizeof(T))))
-      alone.push_back(j);
-    else
-      jobs.push_back(j);
-  }
-  // jobs grouped by radius class, so that each class is one range of tiles
-  // launched with the kernels for its radii
-  std::stable_sort(jobs.begin(), jobs.end(), [&](int a, int b) {
-    return radius_class(int(std::floor(R[a]))) < radius_class(int(std::floor(R[b])));
-  });
-  const int K = int(jobs.size());
-  if (K > 0) {
-    const int caps = ctx->p1_caps;
-    std::vector<int64_t> col_base(K + 1, 0), tile_base(K + 1, 0);
-    int ocap = 0;
-    for (int k = 0; k < K; ++k) {
-      const DevVol& v = vol[jobs[k]];
-      const int rint = int(std::floor(R[jobs[k]]));
-      col_base[k + 1] = col_base[k] + int64_t(v.nx) * v.ny;
-      const Axis ax = axis_for(ctx, v.nx, v.ny);
-      tile_base[k + 1] = tile_base[k] + int64_t(ax.nx) * ((ax.ny + vo::kGatherTile - 1) / vo::kGatherTile);
-      ocap = std::max(ocap, ocap_for(ctx, rint));
-    }
-    std::vector<double> radii(K);
-    std::vector<int64_t> htab_base(K, 0);
-    for (int k = 0; k < K; ++k) {
-      radii[k] = R[jobs[k]];
-      const int64_t w = int64_t(std::floor(R[jobs[k]])) + 1;
-      if (k + 1 < K) htab_base[k + 1] = htab_base[k] + w * w;
-    }
-    std::vector<double> htab_host;
-    const double* htab = upload_htab(ctx, kG_HTAB, radii, htab_host);
-    const int64_t C = col_base[K], NT = tile_base[K];
-    if (NT > INT32_MAX || C + 1 > INT32_MAX) fail(ctx, VO_ERR_ARGUMENT, "batch too large");
-    const int64_t garena = int64_t(vo::kGatherTile) * 2 * ocap;
-    T* m_lo = static_cast<T*>(ws(ctx, kM_LO, size_t(C) * caps * sizeof(T)));
-    T* m_hi = static_cast<T*>(ws(ctx, kM_HI, size_t(C) * caps * sizeof(T)));
-    uint16_t* m_dy = static_cast<uint16_t*>(ws(ctx, kM_DY, size_t(C) * caps * sizeof(uint16_t)));
-    T* s_lo = static_cast<T*>(ws(ctx, kM_SLO, size_t(C) * (caps / vo::kSlotBlock) * sizeof(T)));
-    T* s_hi = static_cast<T*>(ws(ctx, kM_SHI, size_t(C) * (caps / vo::kSlotBlock) * sizeof(T)));
-    T* c_hi = static_cast<T*>(ws(ctx, kM_CHI, size_t(C) * sizeof(T)));
-    int32_t* m_cnt = static_cast<int32_t*>(ws(ctx, kM_CNT, size_t(C) * sizeof(int32_t)));
-    T* o_z = static_cast<T*>(ws(ctx, kO_Z, size_t(NT) * garena * sizeof(T)));
-    int64_t* o_start = static_cast<int64_t*>(ws(ctx, kO_START, size_t(C) * sizeof(int64_t)));
-    int64_t* o_len = static_cast<int64_t*>(ws(ctx, kO_LEN, size_t(C + 1) * sizeof(int64_t)));
-    int* status = static_cast<int*>(ws(ctx, kB_STATUS, size_t(K) * sizeof(int)));
-    const int64_t list_tile = 2 * int64_t(ocap) * vo::kWave;  // doubles of interval lists per tile
-    double* lists = static_cast<double*>(ws(ctx, kG_LISTS, size_t(NT) * list_tile * sizeof(double)));
-    std::vector<vo::Pass1ZArgs<T>> a1(K);
-    std::vector<vo::Pass2GArgs<T>> a2(K);
-    for (int k = 0; k < K; ++k) {
-      const DevVol& v = vol[jobs[k]];
-      const int rint = int(std::floor(R[jobs[k]]));
-      const int64_t cb = col_base[k], sb = cb * caps;
-      const int64_t qb = cb * (caps / vo::kSlotBlock);  // first block summary of this volume
-      const Axis ax = axis_for(ctx, v.nx, v.ny);
-      a1[k] = vo::Pass1ZArgs<T>{ax.nx, ax.ny, rint, v.off, static_cast<const T*>(v.z), m_lo + sb, m_hi + sb, m_dy + sb,
-                                caps, m_cnt + cb, status + k, s_lo + qb, s_hi + qb, c_hi + cb, ax.sx, ax.sy};
-      a2[k] = vo::Pass2GArgs<T>{ax.nx, ax.ny, rint, R[jobs[k]] * R[jobs[k]], m_lo + sb, m_hi + sb, m_dy + sb, caps,
-                                m_cnt + cb, o_z, garena, o_start + cb, o_len + cb, nullptr, status + k, ocap,
-                                htab + htab_base[k], tile_base[k] * garena, lists + tile_base[k] * list_tile,
-                                s_lo + qb, s_hi + qb, c_hi + cb, ax.sx, ax.sy};
-    }
-    const auto* d1 = upload_table(ctx, kB_TAB0, a1);
-    const auto* d2 = upload_table(ctx, kB_TAB1, a2);
-    const int64_t* dt = upload_table(ctx, kB_TAB2, tile_base);
-    check(ctx, hipMemsetAsync(status, 0, size_t(K) * sizeof(int), ctx->stream), "memset status");
-    // columns of a volume whose pass 1 failed, or of a gather tile that
-    // overflowed, are never written: zero lengths keep the batch-wide CSR
-    // assembly from reading them (those volumes are redone alone)
-    check(ctx, hipMemsetAsync(o_len, 0, size_t(C + 1) * sizeof(int64_t), ctx->stream), "memset lengths");
-    check(ctx, hipEventRecord(ctx->ev[0], ctx->stream), "event");
-    // tile range and largest radius of each class
-    int64_t cls_t0[3] = {0, 0, 0}, cls_t1[3] = {0, 0, 0};
-    int cls_rmax[3] = {0, 0, 0};
-    for (int k = K - 1; k >= 0; --k) {
-      const int rint = int(std::floor(R[jobs[k]]));
-      const int c = radius_class(rint);
-      if (cls_t1[c] == 0) cls_t1[c] = tile_base[k + 1];
-      cls_t0[c] = tile_base[k];
-      cls_rmax[c] = std::max(cls_rmax[c], rint);
-    }
-    const dim3 block(static_cast<unsigned>(vo::kWave));
-    for (int c = 0; c < 3; ++c) {
-      if (cls_t1[c] > cls_t0[c])
-        launch_zsweep_batch_events<T>(ctx, events_for(ctx, cls_rmax[c]), cls_rmax[c], cls_t1[c] - cls_t0[c], d1, dt,
-                                      K, cls_t0[c]);
-    }
-    check(ctx, hipGetLastError(), "batched pass1 launch");
-    check(ctx, hipEventRecord(ctx->ev[1], ctx->stream), "event");
-    for (int c = 0; c < 3; ++c) {
-      if (cls_t1[c] <= cls_t0[c]) continue;
-      const dim3 grid(static_cast<unsigned>(cls_t1[c] - cls_t0[c]));
-      if (cls_rmax[c] <= vo::kGatherLdsTableMax)
-        hipLaunchKernelGGL((vo::pass2_gather_batch_kernel<T, true>), grid, block,
-                           size_t(cls_rmax[c] + 1) * (cls_rmax[c] + 1) * sizeof(double), ctx->stream, d2, dt, K,
-                           cls_t0[c]);
-      else
-        hipLaunchKernelGGL((vo::pass2_gather_batch_kernel<T, false>), grid, block, 0, ctx->stream, d2, dt, K,
-                           cls_t0[c]);
-    }
-    check(ctx, hipGetLastError(), "batched pass2 launch");
-    check(ctx, hipEventRecord(ctx->ev[2], ctx->stream), "event");
-    // output CSR over all columns of the batch
-    int64_t* glob = static_cast<int64_t*>(ws(ctx, off_slot, size_t(C + 1) * sizeof(int64_t)));
-    exclusive_scan(ctx, o_len, glob, C + 1);
-    std::vector<int> st(K);
-    check(ctx, hipMemcpyAsync(st.data(), status, size_t(K) * sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
-          "copy status");
-    const int64_t total = read_word(ctx, glob + C);  // synchronises
-    float ms1 = 0, ms2 = 0;
-    check(ctx, hipEventElapsedTime(&ms1, ctx->ev[0], ctx->ev[1]), "elapsed");
-    check(ctx, hipEventElapsedTime(&ms2, ctx->ev[1], ctx->ev[2]), "elapsed");
-    if (t1) *t1 += ms1;
-    if (t2) *t2 += ms2;
-    T* z = static_cast<T*>(ws(ctx, z_slot, size_t(std::max<int64_t>(total, 1)) * sizeof(T)));
-    hipLaunchKernelGGL(vo::gather_kernel<T>, dim3(blocks_for(C, 256)), dim3(256), 0, ctx->stream, C,
-                       static_cast<const T*>(o_z), o_start, o_len, glob, z);
-    check(ctx, hipGetLastError(), "gather launch");
-    for (int k = 0; k < K; ++k) {
-      DevVol& v = vol[jobs[k]];
-      if (st[k] != 0) {
-        alone.push_back(jobs[k]);  // lists overflowed: the single-volume path has the fallbacks
-        continue;
-      }
-      v.off = glob + col_base[k];
-      v.z = z;
-      v.count = -1;
-    }
-  }
-  for (int j : alone) {
-    double a = 0, b = 0;
-    DevVol r = dilate<T>(ctx, vol[j], R[j], kV3_OFF, kV3_Z, &a, &b);
-    if (t1) *t1 += a;
-    if (t2) *t2 += b;
-    vol[j] = keep_alone<T>(ctx, r);
-    ++ctx->last_batch_alone;
-  }
-}
-
-// Ring (grid + 2, complement within [lo, hi]) or crop (grid - 2, complement
-// within [lo, hi] with parity) of the volumes vol[idx[...]], in place.
-template <class T>
-void complement_batch(vo_context* ctx, bool ring, std::vector<DevVol>& vol, const std::vector<int>& idx,
-                      const std::vector<double>& lo, const std::vector<double>& hi, int off_slot, int z_slot) {
-  const int K = int(idx.size());
-  if (K == 0) return;
-  std::vector<int64_t> col_base(K + 1, 0);
-  for (int k = 0; k < K; ++k) {
-    const DevVol& v = vol[idx[k]];
-    if (!ring && (v.nx < 2 || v.ny < 2)) fail(ctx, VO_ERR_ARGUMENT, "crop of a grid smaller than 2x2");
-    col_base[k + 1] = col_base[k] + (ring ? int64_t(v.nx + 2) * (v.ny + 2) : int64_t(v.nx - 2) * (v.ny - 2));
-  }
-  const int64_t C = col_base[K];
-  if (C + 1 > INT32_MAX) fail(ctx, VO_ERR_ARGUMENT, "batch too large");
-  int64_t* len = static_cast<int64_t*>(ws(ctx, kO_LEN, size_t(C + 1) * sizeof(int64_t)));
-  int64_t* glob = static_cast<int64_t*>(ws(ctx, off_slot, size_t(C + 1) * sizeof(int64_t)));
-  std::vector<vo::ColumnJob<T>> t(K);
-  for (int k = 0; k < K; ++k) {
-    const DevVol& v = vol[idx[k]];
-    t[k] = vo::ColumnJob<T>{ring ? v.nx : v.nx - 2, ring ? v.ny : v.ny - 2, v.off, static_cast<const T*>(v.z),
-                            T(lo[idx[k]]), T(hi[idx[k]]), len + col_base[k], glob + col_base[k], nullptr};
-  }
-  const int64_t* dcb = upload_table(ctx, kB_TAB2, col_base);
-  const vo::ColumnJob<T>* dt = upload_table(ctx, kB_TAB0, t);
-  const dim3 grid(blocks_for(std::max<int64_t>(C, 1), 256));
-  auto launch = [&](int phase) {
-    if (ring)
-      hipLaunchKernelGGL(vo::ring_complement_batch_kernel<T>, grid, dim3(256), 0, ctx->stream, dt, dcb, K, phase);
-    else
-      hipLaunchKernelGGL(vo::crop_complement_batch_kernel<T>, grid, dim3(256), 0, ctx->stream, dt, dcb, K, phase);
-    check(ctx, hipGetLastError(), "batched complement launch");
-  };
-  launch(0);
-  check(ctx, hipMemsetAsync(len + C, 0, sizeof(int64_t), ctx->stream), "memset");
-  exclusive_scan(ctx, len, glob, C + 1);
-  const int64_t total = read_word(ctx, glob + C);  // synchronises: the first table upload is done
-  T* z = static_cast<T*>(ws(ctx, z_slot, size_t(std::max<int64_t>(total, 1)) * sizeof(T)));
-  for (auto& e : t) e.out = z;
-  dt = upload_table(ctx, kB_TAB0, t);
-  launch(1);
-  sync(ctx);  // the host table must outlive its upload
-  for (int k = 0; k < K; ++k) {
-    DevVol& v = vol[idx[k]];
-    v.nx = ring ? v.nx + 2 : v.nx - 2;
-    v.ny = ring ? v.ny + 2 : v.ny - 2;
-    v.off = glob + col_base[k];
-    v.z = z;
-    v.count = -1;
-  }
-}
-
 // The programs of the four operations (Voronoi.cpp:8-17, offset3d.cpp:124-133).
 enum BatchStep { kStepNone = -1, kStepRing = 0, kStepDilate = 1, kStepCrop = 2 };
 
@@ -955,33 +766,82 @@ int batch_step(int op, int s) {
   return prog[op][s];
 }
 
-template <class T>
-void run_batch(vo_context* ctx, const std::vector<int>& op, const std::vector<double>& R, std::vector<DevVol>& vol,
-               const std::vector<double>& zlo, const std::vector<double>& zhi, double* t1, double* t2) {
-  const int n = int(op.size());
-  std::vector<double> rlo(n), rhi(n);
-  for (int j = 0; j < n; ++j) {
-    rlo[j] = zlo[j] - 1;
-    rhi[j] = zhi[j] + 1;
-  }
-  for (int s = 0; s < 4; ++s) {
-    std::vector<int> by[3];
-    for (int j = 0; j < n; ++j) {
-      const int k = batch_step(op[j], s);
-      if (k != kStepNone) by[k].push_back(j);
-    }
-    const int base = kB_OUT + s * 3 * 2;
-    complement_batch<T>(ctx, true, vol, by[kStepRing], rlo, rhi, base, base + 1);
-    dilate_batch<T>(ctx, vol, R, by[kStepDilate], base + 2, base + 3, t1, t2);
-    complement_batch<T>(ctx, false, vol, by[kStepCrop], zlo, zhi, base + 4, base + 5);
-  }
-}
-
 void free_batch_extra(vo_context* ctx) {
   if (ctx->batch_extra.empty()) return;
   check(ctx, hipStreamSynchronize(ctx->stream), "sync");
   for (void* p : ctx->batch_extra) (void)hipFree(p);
   ctx->batch_extra.clear();
+}
+
+// ---------------------------------------------------------------------------
+// Batched form as one program (no reference counterpart: it replaces a loop
+// of VoronoiMorpho calls over many volumes, offset3d.cpp:116-136).  Each
+// job's operation is a program over up to four stages of primitives -- ring
+// complement, dilation, crop complement (Voronoi.cpp:8-17,
+// offset3d.cpp:124-133) -- and every primitive runs once per stage for all
+// the jobs that need it.  Every output size is bounded before anything runs
+// (a complement adds at most two endpoints per column; a dilated column holds
+// at most the gather lists' capacity of intervals), so all buffers and all
+// kernel tables are set up front: the program reads the input sizes (one
+// synchronisation), then runs to its end with nothing for the host to wait
+// on.  A batch equal to the previous one (same jobs, pointers and sizes)
+// runs from a HIP graph captured on its second call.  Jobs whose lists
+// overflowed (a status bit) and jobs outside the batched kernels' range
+// (R > 127, empty grids, forced kernel forms) go through the single-volume
+// path afterwards.
+
+// Kernel tables of a program, uploaded in one copy.
+struct BlobBuilder {
+  std::vector<char> host;
+  template <class V>
+  size_t put(const std::vector<V>& v) {
+    const size_t at = (host.size() + 255) & ~size_t(255);
+    host.resize(at + std::max<size_t>(v.size() * sizeof(V), 1));
+    if (!v.empty()) std::memcpy(host.data() + at, v.data(), v.size() * sizeof(V));
+    return at;
+  }
+};
+
+void fill_words(vo_context* ctx, void* p, int64_t words, uint32_t v) {
+  if (words <= 0) return;
+  hipLaunchKernelGGL(vo::fill_kernel, dim3(blocks_for(words, 256)), dim3(256), 0, ctx->stream,
+                     static_cast<uint32_t*>(p), words, v);
+  check(ctx, hipGetLastError(), "fill launch");
+}
+
+// One primitive of one stage for several jobs.
+struct ProgStep {
+  int kind = kStepNone;
+  std::vector<int> jobs;          // dilation: grouped by radius class
+  std::vector<int64_t> col_base;  // output columns
+  std::vector<int64_t> tile_base; // dilation: 64-position tiles
+  int64_t C = 0, NT = 0, zcap = 0;
+  int off_slot = 0, z_slot = 0;
+  int64_t* off = nullptr;  // output CSR over the step's columns (absolute into z)
+  void* z = nullptr;
+  size_t t0 = 0, t1 = 0, t2 = 0, t3 = 0;  // table offsets in the blob
+  int64_t cls_t0[3] = {0, 0, 0}, cls_t1[3] = {0, 0, 0};
+  int cls_rmax[3] = {0, 0, 0};
+};
+
+// Whether job j's dilations fit the batched kernels (grid g at each one).
+bool batchable(vo_context* ctx, int op, double R, int nx, int ny, int tsize) {
+  if (op == VO_OP_NOOP) return true;  // only rebased
+  const int rint = int(std::floor(R));
+  if (rint > 127 || ctx->force_sweep || ctx->pass1_form != 0) return false;
+  for (int s = 0; s < 4; ++s) {
+    const int k = batch_step(op, s);
+    if (k == kStepRing) {
+      nx += 2;
+      ny += 2;
+    } else if (k == kStepCrop) {
+      nx -= 2;
+      ny -= 2;
+    } else if (k == kStepDilate && !vo::zsweep_offsets_fit(ctx->p1_caps, axis_for(ctx, nx, ny).ny, tsize)) {
+      return false;
+    }
+  }
+  return true;
 }
 
 template <class T>
@@ -997,7 +857,8 @@ void morph_batch_device(vo_context* ctx, int n, const vo_job* jobs, vo_volume* o
     const vo_job& jb = jobs[j];
     validate_shape(ctx, &jb.in);
     if (jb.op < VO_OP_NOOP || jb.op > VO_OP_OPENING) fail(ctx, VO_ERR_ARGUMENT, "unknown operation");
-    if (!(jb.radius >= 0) || !std::isfinite(jb.radius)) fail(ctx, VO_ERR_ARGUMENT, "radius must be finite and >= 0");
+    if (!(jb.radius >= 0) || !std::isfinite(jb.radius) || jb.radius > 60000)
+      fail(ctx, VO_ERR_ARGUMENT, "radius must be finite and >= 0");
     if ((jb.op == VO_OP_EROSION || jb.op == VO_OP_OPENING || jb.op == VO_OP_CLOSING) && !(jb.in.z_lo <= jb.in.z_hi))
       fail(ctx, VO_ERR_ARGUMENT, "z_lo must not exceed z_hi");
     op[j] = jb.op;
@@ -1023,35 +884,382 @@ void morph_batch_device(vo_context* ctx, int n, const vo_job* jobs, vo_volume* o
     }
     if (!chk.empty()) enqueue_input_check<T>(ctx, chk, nc, ids);
   }
-  run_batch<T>(ctx, op, R, vol, zlo, zhi, t1, t2);
   if (n == 0) return;
-  // rebase every result's offsets to 0 in one buffer
-  std::vector<vo::FinalJob> fj(n);
-  std::vector<int64_t> entry_base(n + 1, 0);
-  for (int j = 0; j < n; ++j) {
-    const int64_t nc = int64_t(vol[j].nx) * vol[j].ny;
-    fj[j] = vo::FinalJob{vol[j].off, nc, entry_base[j]};
-    entry_base[j + 1] = entry_base[j] + nc + 1;
+  const int tsize = int(sizeof(T));
+  std::vector<char> inprog(n, 0);
+  for (int j = 0; j < n; ++j) inprog[j] = batchable(ctx, op[j], R[j], vol[j].nx, vol[j].ny, tsize) ? 1 : 0;
+
+  // 1. input endpoint counts: the program's one synchronisation before its end
+  if (ctx->batch_host_n < size_t(3 * n + 8)) {
+    if (ctx->batch_host) {
+      check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+      check(ctx, hipHostFree(ctx->batch_host), "hipHostFree");
+      ctx->batch_host = nullptr;
+    }
+    const size_t want = size_t(3 * n + 8) * 2;
+    check(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->batch_host), want * sizeof(int64_t), hipHostMallocDefault),
+          "pinned");
+    ctx->batch_host_n = want;
   }
-  int64_t* rel = static_cast<int64_t*>(ws(ctx, kB_FIN_OFF, size_t(entry_base[n]) * sizeof(int64_t)));
-  int64_t* aux = static_cast<int64_t*>(ws(ctx, kB_FIN_AUX, size_t(2 * n) * sizeof(int64_t)));
-  const vo::FinalJob* dfj = upload_table(ctx, kB_TAB0, fj);
-  const int64_t* deb = upload_table(ctx, kB_TAB2, entry_base);
-  hipLaunchKernelGGL(vo::finalize_offsets_kernel, dim3(blocks_for(entry_base[n], 256)), dim3(256), 0, ctx->stream,
-                     dfj, deb, n, rel, aux, aux + n);
-  check(ctx, hipGetLastError(), "finalize launch");
-  std::vector<int64_t> h(size_t(2 * n));
-  check(ctx, hipMemcpyAsync(h.data(), aux, h.size() * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream), "copy");
+  int64_t* hb = ctx->batch_host;
+  std::vector<int64_t> E(n, 0);
+  {
+    std::vector<vo::FinalJob> cj(n);
+    for (int j = 0; j < n; ++j) cj[j] = vo::FinalJob{vol[j].off, int64_t(vol[j].nx) * vol[j].ny, 0};
+    const vo::FinalJob* dcj = upload_table(ctx, kB_TAB1, cj);
+    int64_t* cnt = static_cast<int64_t*>(ws(ctx, kB_FIN_AUX, size_t(3 * n) * sizeof(int64_t)));
+    hipLaunchKernelGGL(vo::job_counts_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, dcj, n, cnt);
+    check(ctx, hipGetLastError(), "count launch");
+    check(ctx, hipMemcpyAsync(hb, cnt, size_t(n) * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream), "copy");
+    sync(ctx);  // also where a bad input is reported
+    for (int j = 0; j < n; ++j) E[j] = hb[j];
+  }
+
+  // 2. the program's steps, with every output bounded
+  int ocap = 0;
+  for (int j = 0; j < n; ++j)
+    if (inprog[j] && op[j] != VO_OP_NOOP) ocap = std::max(ocap, ocap_for(ctx, int(std::floor(R[j]))));
+  struct Cur {
+    int nx, ny;
+    int64_t E;
+  };
+  std::vector<Cur> cur(n);
+  for (int j = 0; j < n; ++j) cur[j] = Cur{vol[j].nx, vol[j].ny, E[j]};
+  std::vector<ProgStep> steps;
+  int64_t maxC = 0, maxDilC = 0, maxNT = 0;
+  for (int s = 0; s < 4; ++s) {
+    for (int kind : {int(kStepRing), int(kStepDilate), int(kStepCrop)}) {
+      ProgStep st;
+      st.kind = kind;
+      for (int j = 0; j < n; ++j)
+        if (inprog[j] && batch_step(op[j], s) == kind) st.jobs.push_back(j);
+      if (st.jobs.empty()) continue;
+      if (kind == kStepDilate)  // grouped by radius class: one range of tiles per class
+        std::stable_sort(st.jobs.begin(), st.jobs.end(), [&](int a, int b) {
+          return radius_class(int(std::floor(R[a]))) < radius_class(int(std::floor(R[b])));
+        });
+      const int K = int(st.jobs.size());
+      st.col_base.assign(K + 1, 0);
+      st.tile_base.assign(K + 1, 0);
+      for (int k = 0; k < K; ++k) {
+        Cur& c = cur[st.jobs[k]];
+        if (kind == kStepRing) {
+          c.nx += 2;
+          c.ny += 2;
+        } else if (kind == kStepCrop) {
+          c.nx -= 2;
+          c.ny -= 2;
+        }
+        const int64_t cols = int64_t(c.nx) * c.ny;
+        c.E = kind == kStepDilate ? 2 * int64_t(ocap) * cols : c.E + 2 * cols;
+        st.col_base[k + 1] = st.col_base[k] + cols;
+        st.zcap += c.E;
+        if (kind == kStepDilate) {
+          const Axis ax = axis_for(ctx, c.nx, c.ny);
+          st.tile_base[k + 1] = st.tile_base[k] + int64_t(ax.nx) * ((ax.ny + vo::kGatherTile - 1) / vo::kGatherTile);
+        }
+      }
+      st.C = st.col_base[K];
+      st.NT = st.tile_base[K];
+      if (st.C + 1 > INT32_MAX || st.NT > INT32_MAX) fail(ctx, VO_ERR_ARGUMENT, "batch too large");
+      st.off_slot = kB_OUT + (s * 3 + kind) * 2;
+      st.z_slot = st.off_slot + 1;
+      maxC = std::max(maxC, st.C);
+      if (kind == kStepDilate) {
+        maxDilC = std::max(maxDilC, st.C);
+        maxNT = std::max(maxNT, st.NT);
+      }
+      steps.push_back(std::move(st));
+    }
+  }
+
+  // 3. workspace, sized for the largest step (steps run one after the other)
+  const int caps = ctx->p1_caps;
+  const int64_t garena = int64_t(vo::kGatherTile) * 2 * ocap;
+  const int64_t list_tile = 2 * int64_t(ocap) * vo::kWave;
+  for (auto& st : steps) {
+    st.off = static_cast<int64_t*>(ws(ctx, st.off_slot, size_t(st.C + 1) * sizeof(int64_t)));
+    st.z = ws(ctx, st.z_slot, size_t(std::max<int64_t>(st.zcap, 1)) * sizeof(T));
+  }
+  int64_t* len = static_cast<int64_t*>(ws(ctx, kO_LEN, size_t(maxC + 1) * sizeof(int64_t)));
+  int* status = static_cast<int*>(ws(ctx, kB_STATUS, size_t(n) * sizeof(int)));
+  T *m_lo = nullptr, *m_hi = nullptr, *s_lo = nullptr, *s_hi = nullptr, *c_hi = nullptr, *o_z = nullptr;
+  uint16_t* m_dy = nullptr;
+  int32_t* m_cnt = nullptr;
+  int64_t* o_start = nullptr;
+  double* lists = nullptr;
+  if (maxDilC > 0) {
+    m_lo = static_cast<T*>(ws(ctx, kM_LO, size_t(maxDilC) * caps * sizeof(T)));
+    m_hi = static_cast<T*>(ws(ctx, kM_HI, size_t(maxDilC) * caps * sizeof(T)));
+    m_dy = static_cast<uint16_t*>(ws(ctx, kM_DY, size_t(maxDilC) * caps * sizeof(uint16_t)));
+    s_lo = static_cast<T*>(ws(ctx, kM_SLO, size_t(maxDilC) * (caps / vo::kSlotBlock) * sizeof(T)));
+    s_hi = static_cast<T*>(ws(ctx, kM_SHI, size_t(maxDilC) * (caps / vo::kSlotBlock) * sizeof(T)));
+    c_hi = static_cast<T*>(ws(ctx, kM_CHI, size_t(maxDilC) * sizeof(T)));
+    m_cnt = static_cast<int32_t*>(ws(ctx, kM_CNT, size_t(maxDilC) * sizeof(int32_t)));
+    o_z = static_cast<T*>(ws(ctx, kO_Z, size_t(maxNT) * garena * sizeof(T)));
+    o_start = static_cast<int64_t*>(ws(ctx, kO_START, size_t(maxDilC) * sizeof(int64_t)));
+    lists = static_cast<double*>(ws(ctx, kG_LISTS, size_t(maxNT) * list_tile * sizeof(double)));
+  }
+  {  // the scan's temporary storage, for the largest step
+    size_t tmp = 0;
+    check(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, len, len, int(maxC + 1), ctx->stream), "scan size");
+    ws(ctx, kSCAN_TMP, tmp);
+  }
+  // final offsets of every program job, rebased to 0
+  std::vector<int> pj;
+  for (int j = 0; j < n; ++j)
+    if (inprog[j]) pj.push_back(j);
+  const int P = int(pj.size());
+  std::vector<int64_t> entry_base(P + 1, 0);
+  for (int k = 0; k < P; ++k)
+    entry_base[k + 1] = entry_base[k] + int64_t(cur[pj[k]].nx) * cur[pj[k]].ny + 1;
+  int64_t* rel = static_cast<int64_t*>(ws(ctx, kB_FIN_OFF, size_t(std::max<int64_t>(entry_base[P], 1)) * sizeof(int64_t)));
+  int64_t* aux = static_cast<int64_t*>(ws(ctx, kB_FIN_AUX, size_t(3 * n) * sizeof(int64_t)));
+
+  // 4. kernel tables (all pointers known now), in one blob
+  BlobBuilder blob;
+  std::vector<DevVol> v = vol;  // each job's current volume as the steps go
+  for (auto& st : steps) {
+    const int K = int(st.jobs.size());
+    if (st.kind == kStepDilate) {
+      std::vector<double> radii(K), htab;
+      std::vector<int64_t> hbase(K, 0);
+      for (int k = 0; k < K; ++k) {
+        radii[k] = R[st.jobs[k]];
+        const int64_t w = int64_t(std::floor(radii[k])) + 1;
+        if (k + 1 < K) hbase[k + 1] = hbase[k] + w * w;
+      }
+      for (double Rk : radii) {
+        const int rint = int(std::floor(Rk));
+        for (int ad = 0; ad <= rint; ++ad)
+          for (int dy = 0; dy <= rint; ++dy) {
+            const double h = Rk * Rk - double(dy) * double(dy) - double(ad) * double(ad);
+            htab.push_back(h >= 0 ? std::sqrt(h) : -1.0);
+          }
+      }
+      st.t3 = blob.put(htab);
+      std::vector<vo::Pass1ZArgs<T>> a1(K);
+      std::vector<vo::Pass2GArgs<T>> a2(K);
+      for (int k = 0; k < K; ++k) {
+        const int j = st.jobs[k];
+        const DevVol& in = v[j];
+        const int rint = int(std::floor(R[j]));
+        const int64_t cb = st.col_base[k], sb = cb * caps, qb = cb * (caps / vo::kSlotBlock);
+        const Axis ax = axis_for(ctx, in.nx, in.ny);
+        // the H-table pointer is patched to the device blob at upload (index stays relative)
+        a1[k] = vo::Pass1ZArgs<T>{ax.nx, ax.ny, rint, in.off, static_cast<const T*>(in.z), m_lo + sb, m_hi + sb, m_dy + sb,
+                                  caps, m_cnt + cb, status + j, s_lo + qb, s_hi + qb, c_hi + cb, ax.sx, ax.sy};
+        a2[k] = vo::Pass2GArgs<T>{ax.nx, ax.ny, rint, R[j] * R[j], m_lo + sb, m_hi + sb, m_dy + sb, caps,
+                                  m_cnt + cb, o_z, garena, o_start + cb, len + cb, nullptr, status + j, ocap,
+                                  reinterpret_cast<const double*>(st.t3 + size_t(hbase[k]) * sizeof(double)),
+                                  st.tile_base[k] * garena, lists + st.tile_base[k] * list_tile, s_lo + qb, s_hi + qb,
+                                  c_hi + cb, ax.sx, ax.sy};
+        const int c = radius_class(rint);
+        if (st.cls_t1[c] == 0) st.cls_t0[c] = st.tile_base[k];
+        st.cls_t1[c] = st.tile_base[k + 1];
+        st.cls_rmax[c] = std::max(st.cls_rmax[c], rint);
+        v[j].off = st.off + cb;
+        v[j].z = st.z;
+      }
+      st.t0 = blob.put(a1);
+      st.t1 = blob.put(a2);
+      st.t2 = blob.put(st.tile_base);
+    } else {
+      const bool ring = st.kind == kStepRing;
+      std::vector<vo::ColumnJob<T>> t(K);
+      for (int k = 0; k < K; ++k) {
+        const int j = st.jobs[k];
+        const DevVol& in = v[j];
+        const double lo = ring ? zlo[j] - 1 : zlo[j], hi = ring ? zhi[j] + 1 : zhi[j];
+        t[k] = vo::ColumnJob<T>{ring ? in.nx : in.nx - 2, ring ? in.ny : in.ny - 2, in.off, static_cast<const T*>(in.z),
+                                T(lo), T(hi), len + st.col_base[k], st.off + st.col_base[k], static_cast<T*>(st.z)};
+        v[j].nx = ring ? in.nx + 2 : in.nx - 2;
+        v[j].ny = ring ? in.ny + 2 : in.ny - 2;
+        v[j].off = st.off + st.col_base[k];
+        v[j].z = st.z;
+      }
+      st.t0 = blob.put(t);
+      st.t2 = blob.put(st.col_base);
+    }
+  }
+  std::vector<vo::FinalJob> fj(P);
+  for (int k = 0; k < P; ++k) {
+    const int j = pj[k];
+    fj[k] = vo::FinalJob{v[j].off, int64_t(v[j].nx) * v[j].ny, entry_base[k]};
+  }
+  const size_t tfj = blob.put(fj), teb = blob.put(entry_base);
+  char* dblob = static_cast<char*>(ws(ctx, kB_TAB0, blob.host.size()));
+  // the H-table pointers of the pass-2 tables are blob offsets: make them device addresses
+  for (auto& st : steps) {
+    if (st.kind != kStepDilate) continue;
+    auto* a2 = reinterpret_cast<vo::Pass2GArgs<T>*>(blob.host.data() + st.t1);
+    for (size_t k = 0; k < st.jobs.size(); ++k)
+      a2[k].htab = reinterpret_cast<const double*>(dblob + reinterpret_cast<size_t>(a2[k].htab));
+  }
+
+  // 5. the program: enqueued, captured or replayed
+  auto enqueue = [&] {
+    fill_words(ctx, status, n, 0);
+    for (auto& st : steps) {
+      const int K = int(st.jobs.size());
+      const int64_t* dbase = reinterpret_cast<const int64_t*>(dblob + st.t2);
+      if (st.kind == kStepDilate) {
+        const auto* d1 = reinterpret_cast<const vo::Pass1ZArgs<T>*>(dblob + st.t0);
+        const auto* d2 = reinterpret_cast<const vo::Pass2GArgs<T>*>(dblob + st.t1);
+        // columns a failed job never writes keep length 0
+        fill_words(ctx, len, 2 * (st.C + 1), 0);
+        for (int c = 0; c < 3; ++c)
+          if (st.cls_t1[c] > st.cls_t0[c])
+            launch_zsweep_batch_events<T>(ctx, events_for(ctx, st.cls_rmax[c]), st.cls_rmax[c],
+                                          st.cls_t1[c] - st.cls_t0[c], d1, dbase, K, st.cls_t0[c]);
+        check(ctx, hipGetLastError(), "batched pass1 launch");
+        for (int c = 0; c < 3; ++c) {
+          if (st.cls_t1[c] <= st.cls_t0[c]) continue;
+          const dim3 grid(static_cast<unsigned>(st.cls_t1[c] - st.cls_t0[c])), block(vo::kWave);
+          if (st.cls_rmax[c] <= vo::kGatherLdsTableMax)
+            hipLaunchKernelGGL((vo::pass2_gather_batch_kernel<T, true>), grid, block,
+                               size_t(st.cls_rmax[c] + 1) * (st.cls_rmax[c] + 1) * sizeof(double), ctx->stream, d2,
+                               dbase, K, st.cls_t0[c]);
+          else
+            hipLaunchKernelGGL((vo::pass2_gather_batch_kernel<T, false>), grid, block, 0, ctx->stream, d2, dbase, K,
+                               st.cls_t0[c]);
+        }
+        check(ctx, hipGetLastError(), "batched pass2 launch");
+        exclusive_scan(ctx, len, st.off, st.C + 1);
+        if (st.C > 0) {  // (a step over empty grids only has its one offset)
+          hipLaunchKernelGGL(vo::gather_kernel<T>, dim3(blocks_for(st.C, 256)), dim3(256), 0, ctx->stream, st.C,
+                             static_cast<const T*>(o_z), o_start, len, st.off, static_cast<T*>(st.z));
+          check(ctx, hipGetLastError(), "gather launch");
+        }
+      } else {
+        const auto* dt = reinterpret_cast<const vo::ColumnJob<T>*>(dblob + st.t0);
+        const dim3 grid(blocks_for(std::max<int64_t>(st.C, 1), 256));
+        auto launch = [&](int phase) {
+          if (st.kind == kStepRing)
+            hipLaunchKernelGGL(vo::ring_complement_batch_kernel<T>, grid, dim3(256), 0, ctx->stream, dt, dbase, K, phase);
+          else
+            hipLaunchKernelGGL(vo::crop_complement_batch_kernel<T>, grid, dim3(256), 0, ctx->stream, dt, dbase, K,
+                               phase);
+          check(ctx, hipGetLastError(), "batched complement launch");
+        };
+        launch(0);
+        fill_words(ctx, len + st.C, 2, 0);
+        exclusive_scan(ctx, len, st.off, st.C + 1);
+        launch(1);
+      }
+    }
+    if (P > 0) {
+      hipLaunchKernelGGL(vo::finalize_offsets_kernel, dim3(blocks_for(entry_base[P], 256)), dim3(256), 0, ctx->stream,
+                         reinterpret_cast<const vo::FinalJob*>(dblob + tfj),
+                         reinterpret_cast<const int64_t*>(dblob + teb), P, rel, aux, aux + n);
+      check(ctx, hipGetLastError(), "finalize launch");
+    }
+    check(ctx, hipMemcpyAsync(hb, aux, size_t(2 * n) * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream), "copy");
+    check(ctx, hipMemcpyAsync(hb + 2 * n, status, size_t(n) * sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "copy");
+  };
+  // the graph key: the tables (every pointer and parameter the kernels see),
+  // the launch shapes, and the buffers the program itself names
+  std::vector<char> key = blob.host;
+  auto add_key = [&](const void* p, size_t b) { key.insert(key.end(), static_cast<const char*>(p), static_cast<const char*>(p) + b); };
+  for (auto& st : steps) {
+    add_key(&st.C, sizeof(st.C));
+    add_key(&st.NT, sizeof(st.NT));
+    add_key(st.cls_t0, sizeof(st.cls_t0));
+    add_key(st.cls_t1, sizeof(st.cls_t1));
+    add_key(st.cls_rmax, sizeof(st.cls_rmax));
+    add_key(&st.off, sizeof(st.off));
+  }
+  const void* bufs[] = {dblob, hb, len, status, rel, aux, ws(ctx, kSCAN_TMP, 16)};
+  add_key(bufs, sizeof(bufs));
+  for (int c = 0; c < kTuneBuckets; ++c) add_key(&ctx->tuning.events[c], sizeof(int));
+
+  check(ctx, hipMemcpyAsync(dblob, blob.host.data(), blob.host.size(), hipMemcpyHostToDevice, ctx->stream), "upload");
+  check(ctx, hipEventRecord(ctx->ev[0], ctx->stream), "event");
+  BatchGraph& bg = ctx->bgraph;
+  const bool same = bg.key == key;
+  if (same && bg.exec) {
+    check(ctx, hipGraphLaunch(bg.exec, ctx->stream), "graph launch");
+  } else if (same && ctx->use_graphs) {  // second time: capture, then launch
+    bool captured = false;
+    if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+      bool ok = true;
+      try {
+        enqueue();
+      } catch (const Failure&) {
+        ok = false;
+      }
+      hipGraph_t g = nullptr;
+      ok = hipStreamEndCapture(ctx->stream, &g) == hipSuccess && ok;
+      hipGraphExec_t ex = nullptr;
+      ok = ok && g && hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess;
+      if (g) (void)hipGraphDestroy(g);
+      if (ok) {
+        bg.exec = ex;
+        check(ctx, hipGraphLaunch(bg.exec, ctx->stream), "graph launch");
+        captured = true;
+      } else if (ex) {
+        (void)hipGraphExecDestroy(ex);
+      }
+    }
+    if (!captured) {  // this device or runtime would not capture the program: run it as it is from now on
+      (void)hipGetLastError();
+      ctx->error.clear();
+      ctx->use_graphs = false;
+      enqueue();
+    }
+  } else {
+    if (bg.exec) {
+      (void)hipGraphExecDestroy(bg.exec);
+      bg.exec = nullptr;
+    }
+    bg.key = std::move(key);
+    enqueue();
+  }
+  check(ctx, hipEventRecord(ctx->ev[1], ctx->stream), "event");
   sync(ctx);
+  float ms = 0;
+  check(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]), "elapsed");
+  if (t1) *t1 = ms;  // the whole program (passes and complements)
+
+  // 6. results; failed and non-program jobs through the single-volume path
+  const int* st_host = reinterpret_cast<const int*>(hb + 2 * n);
+  std::vector<int64_t> first(n), count(n);
   for (int j = 0; j < n; ++j) {
-    out[j].nx = vol[j].nx;
-    out[j].ny = vol[j].ny;
-    out[j].offsets = rel + entry_base[j];
-    out[j].z = static_cast<const T*>(vol[j].z) + h[j];
+    first[j] = hb[j];
+    count[j] = hb[n + j];
+  }
+  for (int k = 0; k < P; ++k) {
+    const int j = pj[k];
+    out[j].nx = v[j].nx;
+    out[j].ny = v[j].ny;
+    out[j].offsets = rel + entry_base[k];
+    out[j].z = static_cast<const T*>(v[j].z) + first[k];
     out[j].z_lo = zlo[j];
     out[j].z_hi = zhi[j];
-    if (out_count) out_count[j] = h[n + j];
+    if (out_count) out_count[j] = count[k];
   }
+  for (int j = 0; j < n; ++j) {
+    if (inprog[j] && st_host[j] == 0) continue;
+    if (op[j] == VO_OP_NOOP) {
+      out[j] = jobs[j].in;
+      if (out_count) out_count[j] = E[j];
+      continue;
+    }
+    double a = 0, b = 0;
+    DevVol r = run_op<T>(ctx, op[j], vol[j], R[j], zlo[j], zhi[j], &a, &b);
+    if (t1) *t1 += a;
+    if (t2) *t2 += b;
+    DevVol kept = keep_alone<T>(ctx, r);
+    out[j].nx = kept.nx;
+    out[j].ny = kept.ny;
+    out[j].offsets = kept.off;
+    out[j].z = kept.z;
+    out[j].z_lo = zlo[j];
+    out[j].z_hi = zhi[j];
+    if (out_count) out_count[j] = kept.count;
+    ++ctx->last_batch_alone;
+  }
+  sync(ctx);
 }
 
 // ---------------------------------------------------------------------------
@@ -1520,6 +1728,7 @@ int vo_create(vo_context** out, int device) {
   if (const char* e = std::getenv("VO_PASS2")) ctx->force_sweep = std::string(e) == "sweep";
   if (const char* e = std::getenv("VO_ZSWEEP_EVENTS")) ctx->zsweep_events = std::atoi(e);
   if (const char* e = std::getenv("VO_INJECT_HIP_ERROR")) ctx->inject = e;
+  if (const char* e = std::getenv("VO_BATCH_GRAPH")) ctx->use_graphs = std::string(e) != "0";
   if (const char* e = std::getenv("VO_AXIS")) ctx->axis_forced = std::string(e) == "x" ? 1 : (std::string(e) == "y" ? 0 : -1);
   if (const char* e = std::getenv("VO_PASS1"))
     ctx->pass1_form = std::string(e) == "cull" ? 2 : (std::string(e) == "paint" ? 1 : 0);
@@ -1562,6 +1771,8 @@ void vo_destroy(vo_context* ctx) {
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->host_words) (void)hipHostFree(ctx->host_words);
+  if (ctx->batch_host) (void)hipHostFree(ctx->batch_host);
+  if (ctx->bgraph.exec) (void)hipGraphExecDestroy(ctx->bgraph.exec);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

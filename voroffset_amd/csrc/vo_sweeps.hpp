@@ -794,10 +794,53 @@ __device__ void check_column(const InputCheck<T>& g, int64_t c, int job, int* wo
   }
 }
 
+// One volume: a wavefront checks 64 consecutive columns, whose endpoints are
+// one contiguous range, reading it with coalesced loads 64 endpoints at a
+// time; the columns starting inside each such chunk are one 64-bit mask (an
+// OR over the lanes), so that an endpoint is compared with the one before it
+// only within its column.  For the ranged operations every endpoint is
+// checked against [lo, hi] (with the columns sorted that is the same as
+// checking their first and last).
 template <class T>
-__global__ void check_input_kernel(InputCheck<T> g, int64_t ncols, int* word) {
-  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (c < ncols) check_column(g, c, 0, word);
+__global__ __launch_bounds__(256) void check_input_kernel(InputCheck<T> g, int64_t ncols, int* word) {
+  const int L = lane_id() & (kWave - 1);
+  const int64_t c0 = (int64_t(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave) * kWave;
+  if (c0 >= ncols) return;
+  const int64_t c = c0 + L;
+  const bool has = c < ncols;
+  const int64_t s = has ? g.off[c] : 0, e = has ? g.off[c + 1] : 0;
+  int bad = (has && (e < s || ((e - s) & 1))) ? kInputMalformed : 0;
+  // the wavefront's range of endpoints; with malformed offsets only the
+  // per-column checks run
+  const int last = int(min<int64_t>(ncols - c0, kWave)) - 1;
+  const int64_t S = __shfl(s, 0), E = __shfl(e, last);
+  if (ballot(bad != 0) == 0 && E > S) {
+    T carry = T(0);
+    for (int64_t base = S; base < E; base += kWave) {
+      const int64_t i = base + L;
+      const bool in = i < E;
+      const T v = in ? g.z[i] : T(0);
+      // bit k: an endpoint base + k starts a (non-empty) column
+      const uint64_t mine = (has && s < e && s >= base && s < base + kWave) ? (uint64_t(1) << (s - base)) : 0;
+      uint64_t starts = mine;
+#pragma unroll
+      for (int d = 1; d < kWave; d <<= 1) starts |= __shfl_xor(starts, d);
+      T pv = __shfl_up(v, 1);
+      pv = L == 0 ? carry : pv;
+      const bool first = ((starts >> L) & 1) != 0 || i == S;
+      int b = 0;
+      if (in) {
+        if (!isfinite(double(v)) || (!first && v < pv)) b = kInputMalformed;
+        else if (g.range && (double(v) < g.lo || double(v) > g.hi)) b = kInputOutOfRange;
+      }
+      bad |= b;
+      carry = __shfl(v, kWave - 1);
+    }
+  }
+  if (bad) {
+    atomicOr(word, bad);
+    atomicMin(word + 1, 0);
+  }
 }
 
 // Batched: job j owns the columns [col_base[j], col_base[j + 1]).
@@ -1206,8 +1249,8 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
     if (ballot(want) == 0) return;  // the usual case inside a run
     const bool fast = s <= chi && e >= clo && s > below && e < above;
     const bool f = want && fast;
-    clo = f ? fmin(clo, s) : clo;
-    chi = f ? fmax(chi, e) : chi;
+    clo = f && s < clo ? s : clo;
+    chi = f && e > chi ? e : chi;
     dirty = f ? 1 : dirty;
     const bool slow = want && !fast;
     if (ballot(slow && held != 0) != 0) {
@@ -1234,26 +1277,26 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
     load_block(g.m_hi + o, bk.vb);
     bk.q = *reinterpret_cast<const uint4*>(g.m_dy + o);
   };
+  // min / max as compare and select: no operand is ever NaN (checked input,
+  // table heights), so the quieting that fmin / fmax carry is not needed
+  auto dmin = [](double a, double b) { return b < a ? b : a; };
+  auto dmax = [](double a, double b) { return b > a ? b : a; };
   auto compute_block = [&](const Block& bk, int jb, int m, const double* hrow, int& run, double& rs, double& re) {
     const T* va = bk.va;
     const T* vb = bk.vb;
-    int vdy[B];
-    {
-      const uint32_t w[4] = {bk.q.x, bk.q.y, bk.q.z, bk.q.w};
-#pragma unroll
-      for (int u = 0; u < B; ++u) vdy[u] = int((w[u >> 1] >> (16 * (u & 1))) & 0xffffu);
-    }
+    const uint32_t w[4] = {bk.q.x, bk.q.y, bk.q.z, bk.q.w};
+    const int nv = ok ? min(max(m - jb, 0), B) : 0;  // the block's slots holding triples
 #pragma unroll
     for (int u = 0; u < B; ++u) {
-      const int dy = min(vdy[u], rint);
-      const double h = hrow[dy];
-      const bool valid = ok && jb + u < m && vdy[u] <= rint && h >= 0;
+      const int dyr = int((w[u >> 1] >> (16 * (u & 1))) & 0xffffu);
+      const double h = hrow[min(dyr, rint)];
+      const bool valid = u < nv && dyr <= rint && h >= 0;
       const double s = double(va[u]) - h, e = double(vb[u]) + h;
       // triples are z-sorted, so e >= rs whenever s <= re: the run stays connected
       const bool ext = valid && run && s <= re;
       unite(valid && run && !ext, rs, re);
-      rs = ext ? fmin(rs, s) : (valid ? s : rs);
-      re = ext ? fmax(re, e) : (valid ? e : re);
+      rs = ext ? dmin(rs, s) : (valid ? s : rs);
+      re = ext ? dmax(re, e) : (valid ? e : re);
       run = valid ? 1 : run;
     }
   };

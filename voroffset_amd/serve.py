@@ -92,8 +92,8 @@ def footprint_bytes(ncols, endpoints, op, storage_bytes=4, caps=96, ocap=24):
     summaries (lo and hi per 8 slots) and last hi, the triple count, the
     gather's interval lists (2 * ocap doubles whatever the storage), its
     output arena (2 * ocap endpoints), the start, length and offset words;
-    plus the intermediate volumes of the job's stages (complements grow the
-    grid by a ring).  ocap is the largest list capacity a batch uses (24 when
+    plus the outputs of the job's stages at their bounds (the batch program
+    sizes them before it runs).  ocap is the largest list capacity a batch uses (24 when
     it holds a radius below 6, 16 otherwise)."""
     if op not in _DILATIONS:
         raise ValueError(f"unknown operation {op!r}")
@@ -101,9 +101,13 @@ def footprint_bytes(ncols, endpoints, op, storage_bytes=4, caps=96, ocap=24):
         return 0
     per_col = (caps * (2 * storage_bytes + 2) + 2 * (caps // 8) * storage_bytes + storage_bytes + 4
                + 2 * ocap * 8 + 2 * ocap * storage_bytes + 8 + 8 + 8)
-    stages = 4 if op in ("closing", "opening") else (3 if op == "erosion" else 1)
+    dilations = _DILATIONS[op]
+    complements = {"dilation": 0, "erosion": 2, "closing": 2, "opening": 2}[op]
+    # stage outputs, each sized for its bound: a dilation's column holds at
+    # most ocap intervals, a complement adds at most two endpoints a column
     volume = (ncols + 1) * 8 + (endpoints + 2 * ncols) * storage_bytes
-    return ncols * per_col + stages * volume
+    dilated = (ncols + 1) * 8 + 2 * ocap * ncols * storage_bytes
+    return ncols * per_col + dilations * dilated + complements * volume
 
 
 class MorphoServer:
