@@ -293,21 +293,28 @@ std::string status_text(int s) {
   return "a sweep exceeded its LDS capacity:" + t;
 }
 
-// H(|d|, dy) = sqrt(R^2 - d^2 - dy^2) for the gather form, -1 where out of
-// reach; one (rint + 1)^2 table per radius, concatenated, uploaded to `slot`.
-// `host` keeps the values alive until the upload has run (the caller
-// synchronises before it goes out of scope).  Returns the device base.
+// Appends the gather form's table for radius R: rint + 1 rows (|d|) of
+// htab_width(rint) entries, H(|d|, dy) = sqrt(R^2 - d^2 - dy^2) for
+// dy = 0 .. rint, NaN where that is negative, then the NaN entry that slots
+// without a triple read.
+void append_htab(double R, std::vector<double>& out) {
+  const int rint = int(std::floor(R));
+  const double R2 = R * R;
+  for (int ad = 0; ad <= rint; ++ad) {
+    for (int dy = 0; dy <= rint; ++dy) {
+      const double v = R2 - double(dy) * double(dy) - double(ad) * double(ad);
+      out.push_back(v >= 0 ? std::sqrt(v) : std::nan(""));
+    }
+    out.push_back(std::nan(""));
+  }
+}
+
+// The tables of `radii`, concatenated, uploaded to `slot`.  `host` keeps the
+// values alive until the upload has run (the caller synchronises before it
+// goes out of scope).  Returns the device base.
 const double* upload_htab(vo_context* ctx, int slot, const std::vector<double>& radii, std::vector<double>& host) {
   host.clear();
-  for (double R : radii) {
-    const int rint = int(std::floor(R));
-    const double R2 = R * R;
-    for (int ad = 0; ad <= rint; ++ad)
-      for (int dy = 0; dy <= rint; ++dy) {
-        const double v = R2 - double(dy) * double(dy) - double(ad) * double(ad);
-        host.push_back(v >= 0 ? std::sqrt(v) : -1.0);
-      }
-  }
+  for (double R : radii) append_htab(R, host);
   double* d = static_cast<double*>(ws(ctx, slot, host.size() * sizeof(double)));
   check(ctx, hipMemcpyAsync(d, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream),
         "upload H table");
@@ -496,7 +503,7 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
                              ax.sx, ax.sy};
         if (rint <= vo::kGatherLdsTableMax)
           hipLaunchKernelGGL((vo::pass2_gather_kernel<T, true>), dim3(unsigned(ntiles)), dim3(vo::kWave),
-                             size_t(rint + 1) * (rint + 1) * sizeof(double), ctx->stream, ag);
+                             size_t(rint + 1) * vo::htab_width(rint) * sizeof(double), ctx->stream, ag);
         else
           hipLaunchKernelGGL((vo::pass2_gather_kernel<T, false>), dim3(unsigned(ntiles)), dim3(vo::kWave), 0,
                              ctx->stream, ag);
@@ -1026,17 +1033,10 @@ void morph_batch_device(vo_context* ctx, int n, const vo_job* jobs, vo_volume* o
       std::vector<int64_t> hbase(K, 0);
       for (int k = 0; k < K; ++k) {
         radii[k] = R[st.jobs[k]];
-        const int64_t w = int64_t(std::floor(radii[k])) + 1;
-        if (k + 1 < K) hbase[k + 1] = hbase[k] + w * w;
+        const int rk = int(std::floor(radii[k]));
+        if (k + 1 < K) hbase[k + 1] = hbase[k] + int64_t(rk + 1) * vo::htab_width(rk);
       }
-      for (double Rk : radii) {
-        const int rint = int(std::floor(Rk));
-        for (int ad = 0; ad <= rint; ++ad)
-          for (int dy = 0; dy <= rint; ++dy) {
-            const double h = Rk * Rk - double(dy) * double(dy) - double(ad) * double(ad);
-            htab.push_back(h >= 0 ? std::sqrt(h) : -1.0);
-          }
-      }
+      for (double Rk : radii) append_htab(Rk, htab);
       st.t3 = blob.put(htab);
       std::vector<vo::Pass1ZArgs<T>> a1(K);
       std::vector<vo::Pass2GArgs<T>> a2(K);
@@ -1118,7 +1118,8 @@ void morph_batch_device(vo_context* ctx, int n, const vo_job* jobs, vo_volume* o
           const dim3 grid(static_cast<unsigned>(st.cls_t1[c] - st.cls_t0[c])), block(vo::kWave);
           if (st.cls_rmax[c] <= vo::kGatherLdsTableMax)
             hipLaunchKernelGGL((vo::pass2_gather_batch_kernel<T, true>), grid, block,
-                               size_t(st.cls_rmax[c] + 1) * (st.cls_rmax[c] + 1) * sizeof(double), ctx->stream, d2,
+                               size_t(st.cls_rmax[c] + 1) * vo::htab_width(st.cls_rmax[c]) * sizeof(double),
+                               ctx->stream, d2,
                                dbase, K, st.cls_t0[c]);
           else
             hipLaunchKernelGGL((vo::pass2_gather_batch_kernel<T, false>), grid, block, 0, ctx->stream, d2, dbase, K,

@@ -1092,7 +1092,7 @@ struct Pass2GArgs {
   int* tile_list;  // [0] = count, then the ids of tiles to recompute with the sweep form
   int* status;
   int ocap;             // intervals per output column list
-  const double* htab;   // H by (|d|, dy): htab[|d| * (rint + 1) + dy], -1 out of reach
+  const double* htab;   // H by (|d|, dy): htab[|d| * htab_width(rint) + dy], NaN out of reach
   int64_t o_base;       // first endpoint of this volume's tiles in o_z (batched launches)
   double* lists;        // interval lists, 2 * ocap * 64 doubles per tile (lane-minor)
   const T* s_lo;        // block summaries from pass 1 (null: none, no block is skipped)
@@ -1162,10 +1162,32 @@ __device__ __forceinline__ void load_block(const double* p, double (&v)[kSlotBlo
 // Source columns whose ranges the gather checks per memory round trip.
 constexpr int kGatherGroup = 4;
 
-// Largest radius whose H table is staged in LDS ((R + 1)^2 doubles, 8 KB);
-// above it the lanes read the table from global memory (L1-resident).
+// Largest radius whose H table is staged in LDS ((R + 1)(R + 2) doubles,
+// 8.25 KB); above it the lanes read the table from global memory
+// (L1-resident).
 constexpr int kGatherLdsTableMax = 31;
+
+// The gather's H table: row |d| holds H(|d|, dy) = sqrt(R^2 - d^2 - dy^2)
+// for dy = 0 .. rint, NaN where that is negative, and one more NaN entry
+// (dy = rint + 1) that slots without a triple are pointed at.
+__host__ __device__ constexpr int htab_width(int rint) { return rint + 2; }
 extern __shared__ double vo_htab_lds[];
+
+// min / max of two doubles as one v_min_f64 / v_max_f64 each.  With one
+// operand a quiet NaN they return the other operand; every NaN the gather
+// makes is quiet (the table's, and sums with it).  Written as instructions
+// because the library calls (fmin / fmax) add canonicalising maxima of loop-
+// carried operands on every use.
+__device__ __forceinline__ double qmin(double a, double b) {
+  double r;
+  asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ double qmax(double a, double b) {
+  double r;
+  asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
 
 // One tile = (x, y0 .. y0 + 63); consecutive tiles are consecutive x, whose
 // source windows overlap, so the launches hand them to one XCD (xcd_tile).
@@ -1188,7 +1210,7 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
   const bool active = y < g.ny;
   const double* htab = g.htab;
   if constexpr (kLdsTable) {
-    for (int i = L; i < (rint + 1) * (rint + 1); i += kWave) vo_htab_lds[i] = g.htab[i];
+    for (int i = L; i < (rint + 1) * htab_width(rint); i += kWave) vo_htab_lds[i] = g.htab[i];
     wave_sync();
     htab = vo_htab_lds;
   }
@@ -1265,8 +1287,8 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
   // The lane's block of B contiguous triples of column c from jb on (jb is a
   // multiple of B and below caps, so the whole block is inside the column's
   // slots; values past the column's count are read and discarded): dilated
-  // and fused into the open run, which is united when a gap opens.  The loop
-  // is written with selects, so it has no branches.
+  // and fused into the open run, which is united when a gap opens.  The
+  // wavefront branches only when some lane's run ends.
   struct Block {
     T va[B], vb[B];
     uint4 q;  // the B 16-bit line offsets
@@ -1277,37 +1299,54 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
     load_block(g.m_hi + o, bk.vb);
     bk.q = *reinterpret_cast<const uint4*>(g.m_dy + o);
   };
-  // min / max as compare and select: no operand is ever NaN (checked input,
-  // table heights), so the quieting that fmin / fmax carry is not needed
-  auto dmin = [](double a, double b) { return b < a ? b : a; };
-  auto dmax = [](double a, double b) { return b > a ? b : a; };
-  auto compute_block = [&](const Block& bk, int jb, int m, const double* hrow, int& run, double& rs, double& re) {
+  // A lane's open run [rs, re] starts as (NaN, NaN).  Slots without a triple
+  // (past the column's count, or every slot of a block the lane does not
+  // need) read the table's NaN column, as do triples out of reach: their s
+  // and e are NaN, no comparison with them holds, and qmin / qmax return the
+  // other operand, so they change nothing and need no test of their own.
+  const double qnan = __builtin_nan("");
+  const int hsent = rint + 1;
+  auto compute_block = [&](const Block& bk, int jb, int m, const double* hrow, double& rs, double& re) {
     const T* va = bk.va;
     const T* vb = bk.vb;
-    const uint32_t w[4] = {bk.q.x, bk.q.y, bk.q.z, bk.q.w};
     const int nv = ok ? min(max(m - jb, 0), B) : 0;  // the block's slots holding triples
+    // the 16-bit offsets of slots u >= nv set to 0xffff, which clamps to hsent
+    const uint64_t q0 = (uint64_t(bk.q.y) << 32 | bk.q.x) | (nv >= 4 ? 0 : ~uint64_t(0) << (16 * nv));
+    const uint64_t q1 = (uint64_t(bk.q.w) << 32 | bk.q.z) |
+                        (nv >= 8 ? 0 : nv <= 4 ? ~uint64_t(0) : ~uint64_t(0) << (16 * (nv - 4)));
+    auto hread = [&](int u) { return hrow[min(int(((u < 4 ? q0 : q1) >> (16 * (u & 3))) & 0xffffu), hsent)]; };
+    // with FP32 storage the block's table reads go first, in flight together
+    // (FP64 storage has no registers left for that)
+    constexpr bool kAhead = sizeof(T) == 4;
+    double hv[B];
+    if constexpr (kAhead) {
+#pragma unroll
+      for (int u = 0; u < B; ++u) hv[u] = hread(u);
+    }
 #pragma unroll
     for (int u = 0; u < B; ++u) {
-      const int dyr = int((w[u >> 1] >> (16 * (u & 1))) & 0xffffu);
-      const double h = hrow[min(dyr, rint)];
-      const bool valid = u < nv && dyr <= rint && h >= 0;
+      const double h = kAhead ? hv[u] : hread(u);
       const double s = double(va[u]) - h, e = double(vb[u]) + h;
-      // triples are z-sorted, so e >= rs whenever s <= re: the run stays connected
-      const bool ext = valid && run && s <= re;
-      unite(valid && run && !ext, rs, re);
-      rs = ext ? dmin(rs, s) : (valid ? s : rs);
-      re = ext ? dmax(re, e) : (valid ? e : re);
-      run = valid ? 1 : run;
+      // triples are z-sorted, so e >= rs whenever s <= re: the run stays
+      // connected until a triple starts above it
+      const bool gap = s > re;
+      if (ballot(gap) != 0) {
+        unite(gap, rs, re);
+        rs = gap ? qnan : rs;
+        re = gap ? qnan : re;
+      }
+      rs = qmin(rs, s);
+      re = qmax(re, e);
     }
   };
   // lanes that do not need the block (its range is inside their cached
   // interval) neither load nor use it
-  auto process_block = [&](int c, int jb, int m, const double* hrow, int& run, double& rs, double& re,
-                           bool need) {
+  auto process_block = [&](int c, int jb, int m, const double* hrow, double& rs, double& re, bool need) {
     Block bk = {};
     if (need) load_item(c, jb, bk);
-    compute_block(bk, jb, need ? m : jb, hrow, run, rs, re);
+    compute_block(bk, jb, need ? m : jb, hrow, rs, re);
   };
+  const int W = htab_width(rint);
   if (g.s_lo) {
     // Source columns nearest first (d = 0, -1, +1, -2, ...), their blocks in
     // z order.  The near columns build most of the union; a farther column
@@ -1346,7 +1385,7 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
 #pragma unroll
       for (int j = 0; j < G; ++j) {
         const int ad = (grp * G + j + 1) >> 1;
-        const double hmax = ad <= rint ? htab[ad * (rint + 1)] : 0.0;
+        const double hmax = ad <= rint ? htab[ad * W] : 0.0;
         const bool in = pm[j] == 0 || ok == 0 || (double(plo[j]) - hmax >= clo && double(phi[j]) + hmax <= chi);
         need |= ballot(!in) != 0 ? 1u << j : 0u;
       }
@@ -1358,10 +1397,9 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
         const int c = col_of_step(step);
         const int m = count_of(c);  // loaded with the first summary
         const int ad = (step + 1) >> 1;
-        const double* hrow = htab + ad * (rint + 1);
+        const double* hrow = htab + ad * W;
         const double hmax = hrow[0];
-        int run = 0;
-        double rs = 0, re = 0;
+        double rs = qnan, re = qnan;
         int jb = 0;
         T blo = g.s_lo[summary_index(g.ny, g.caps, c, yc, 0)];
         T bhi = g.s_hi[summary_index(g.ny, g.caps, c, yc, 0)];
@@ -1373,13 +1411,13 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
             nbhi = g.s_hi[summary_index(g.ny, g.caps, c, yc, jb + B)];
           }
           const bool inside = jb >= m || ok == 0 || (double(blo) - hmax >= clo && double(bhi) + hmax <= chi);
-          if (ballot(!inside) != 0) process_block(c, jb, m, hrow, run, rs, re, !inside);
+          if (ballot(!inside) != 0) process_block(c, jb, m, hrow, rs, re, !inside);
           if (!more) break;
           jb += B;
           blo = nblo;
           bhi = nbhi;
         }
-        unite(run && ok, rs, re);
+        unite(rs == rs && ok, rs, re);
         flush_held();
       }
     }
@@ -1395,10 +1433,9 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
         if (c < 0 || c >= g.nx) continue;  // uniform
         const int m = count_of(c);
         if (ballot(m > jb && ok) == 0) continue;
-        int run = 0;
-        double rs = 0, re = 0;
-        process_block(c, jb, m, htab + ad * (rint + 1), run, rs, re, true);
-        unite(run && ok, rs, re);
+        double rs = qnan, re = qnan;
+        process_block(c, jb, m, htab + ad * W, rs, re, true);
+        unite(rs == rs && ok, rs, re);
         flush_held();
       }
     }
@@ -2202,12 +2239,45 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
   for (int b = L; b < NB; b += kWave) total += sm.bin[b];
 #pragma unroll
   for (int d = 1; d < kWave; d <<= 1) total += __shfl_xor(total, d);
+  int maxbin = 0;
+  for (int b = L; b < NB; b += kWave) maxbin = max(maxbin, sm.bin[b]);
+  maxbin = wave_max(maxbin);
+  // Split by prefix: with S = ECAP - (largest bin) > (largest bin), slab k
+  // holds the bins whose exclusive prefix lies in [k S, (k + 1) S), at most
+  // S - 1 + (largest bin) < ECAP endpoints, and consecutive bins are in the
+  // same or the next slab; the boundaries come from one scan of the bins.
+  const int S = ECAP - maxbin;
+  const bool by_prefix = total > ECAP && S > maxbin && (total - 1) / S + 1 <= kMaxSlabs;
   if (total <= ECAP) {
     if (L == 0) {
       sm.slab_end[0] = NB;
       sm.slab_end[kMaxSlabs] = 1;
     }
-  } else if (L == 0) {
+  } else if (by_prefix) {
+    // trailing empty bins (exclusive prefix == total) join the last slab
+    const int last_slab = (total - 1) / S;
+    int carry = 0, prev_last = 0;
+    for (int base = 0; base < NB; base += kWave) {
+      const int b = base + L;
+      const int c = b < NB ? sm.bin[b] : 0;
+      int v = c;
+#pragma unroll
+      for (int d = 1; d < kWave; d <<= 1) {
+        const int o = __shfl_up(v, d);
+        if (L >= d) v += o;
+      }
+      const int slab = min((carry + v - c) / S, last_slab);
+      int prev = __shfl_up(slab, 1);
+      prev = L == 0 ? prev_last : prev;
+      if (b < NB && b > 0 && slab != prev) sm.slab_end[prev] = b;
+      prev_last = __shfl(slab, kWave - 1);
+      carry += __shfl(v, kWave - 1);
+    }
+    if (L == 0) {
+      sm.slab_end[prev_last] = NB;
+      sm.slab_end[kMaxSlabs] = prev_last + 1;
+    }
+  } else if (L == 0) {  // greedy, one bin at a time
     int nslab = 0, acc = 0;
     for (int b = 0; b < NB && nslab >= 0; ++b) {
       const int c = sm.bin[b];
