@@ -439,6 +439,7 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
   int chunk2 = Shapes<T>::p2_cmax;
   bool zsweep_failed = false;  // a window too dense for the z-sweep form: sweep along y instead
   int64_t fast_total = -1;     // the output size, when the gather's one synchronisation settled it
+  int64_t gather_end = 0;      // the gather form's part of the output arena (interleaved columns)
   for (int attempt = 0;; ++attempt) {
     const int nch1 = (ny + chunk1 - 1) / chunk1;
     const int64_t grid1 = int64_t(nx) * nch1;
@@ -543,6 +544,7 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
       check(ctx, hipGetLastError(), "pass2 launch");
     }
     ctx->last_fallback_tiles = nfall;
+    gather_end = ntiles * garena;
     if (fast_total >= 0) {
       if (t1 || t2) {
         float ms1 = 0, ms2 = 0;
@@ -591,7 +593,7 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
   }
   T* oz = static_cast<T*>(ws(ctx, out_z_slot, size_t(std::max<int64_t>(total, 1)) * sizeof(T)));
   hipLaunchKernelGGL(vo::gather_kernel<T>, dim3(blocks_for(ncols, 256)), dim3(256), 0, ctx->stream, ncols,
-                     static_cast<const T*>(slot_ptr(ctx, kO_Z)), o_start, o_len, out_off, oz);
+                     static_cast<const T*>(slot_ptr(ctx, kO_Z)), o_start, o_len, out_off, oz, gather_end);
   check(ctx, hipGetLastError(), "gather launch");
   out.z = oz;
   out.count = total;
@@ -1129,7 +1131,7 @@ void morph_batch_device(vo_context* ctx, int n, const vo_job* jobs, vo_volume* o
         exclusive_scan(ctx, len, st.off, st.C + 1);
         if (st.C > 0) {  // (a step over empty grids only has its one offset)
           hipLaunchKernelGGL(vo::gather_kernel<T>, dim3(blocks_for(st.C, 256)), dim3(256), 0, ctx->stream, st.C,
-                             static_cast<const T*>(o_z), o_start, len, st.off, static_cast<T*>(st.z));
+                             static_cast<const T*>(o_z), o_start, len, st.off, static_cast<T*>(st.z), INT64_MAX);
           check(ctx, hipGetLastError(), "gather launch");
         }
       } else {

@@ -743,14 +743,25 @@ __global__ __launch_bounds__(kWave) void pass2_kernel(Pass2Args<T> g) {
 // ---------------------------------------------------------------------------
 
 // Gather per-column results (start, count) into CSR at precomputed offsets.
+// A column whose start is below `interleaved` was written by the gather form
+// of pass 2: its interval i is at start + i * kOutStride (the wavefront's
+// i-th intervals side by side); the others are contiguous.
+constexpr int kOutStride = 2 * kWave;
 template <class T>
 __global__ void gather_kernel(int64_t ncols, const T* src, const int64_t* start, const int64_t* cnt,
-                              const int64_t* off, T* dst) {
+                              const int64_t* off, T* dst, int64_t interleaved) {
   const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (c >= ncols) return;
   const int64_t s = start[c], o = off[c];
   const int64_t n = cnt[c];
-  for (int64_t i = 0; i < n; ++i) dst[o + i] = src[s + i];
+  if (s < interleaved) {
+    for (int64_t i = 0; i < n; i += 2) {
+      dst[o + i] = src[s + (i >> 1) * kOutStride];
+      dst[o + i + 1] = src[s + (i >> 1) * kOutStride + 1];
+    }
+  } else {
+    for (int64_t i = 0; i < n; ++i) dst[o + i] = src[s + i];
+  }
 }
 
 // Device-side check of the CompressedVolume invariants the kernels rely on
@@ -1400,7 +1411,23 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
         const double* hrow = htab + ad * W;
         const double hmax = hrow[0];
         double rs = qnan, re = qnan;
-        int jb = 0;
+        // The checks mark the blocks each lane needs (bit k of `want`: block
+        // jw + k); a round then processes every lane's next marked block, so
+        // lanes needing different blocks of the column share a round.  Each
+        // lane's blocks go in z order, so its run continues across them; the
+        // marks of a window of 64 blocks see the cached interval from before
+        // the window's blocks are processed (a block may be processed that
+        // could have been skipped, none that is needed is skipped).
+        uint64_t want = 0;
+        int jb = 0, jw = 0;
+        auto drain = [&]() {
+          while (ballot(want != 0) != 0) {
+            const bool has = want != 0;
+            const int k = has ? __builtin_ctzll(want) : 0;
+            want &= want - 1;
+            process_block(c, jw + k * B, m, hrow, rs, re, has);
+          }
+        };
         T blo = g.s_lo[summary_index(g.ny, g.caps, c, yc, 0)];
         T bhi = g.s_hi[summary_index(g.ny, g.caps, c, yc, 0)];
         for (;;) {
@@ -1411,7 +1438,11 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
             nbhi = g.s_hi[summary_index(g.ny, g.caps, c, yc, jb + B)];
           }
           const bool inside = jb >= m || ok == 0 || (double(blo) - hmax >= clo && double(bhi) + hmax <= chi);
-          if (ballot(!inside) != 0) process_block(c, jb, m, hrow, rs, re, !inside);
+          want |= inside ? 0 : uint64_t(1) << ((jb - jw) / B);
+          if (!more || jb - jw == 63 * B) {
+            drain();
+            jw = jb + B;
+          }
           if (!more) break;
           jb += B;
           blo = nblo;
@@ -1454,10 +1485,12 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
   }
   if (!active) return;
   const int64_t col = int64_t(x) * g.sx + int64_t(y) * g.sy;
-  const int64_t ob = g.o_base + tile * g.arena + int64_t(L) * 2 * ocap;
+  // interval i of lane L at (tile base) + i * kOutStride + 2 L: the
+  // wavefront's i-th intervals are one contiguous run
+  const int64_t ob = g.o_base + tile * g.arena + 2 * int64_t(L);
   for (int i = 0; i < n; ++i) {
-    g.o_z[ob + 2 * i] = T(olo[i * kWave]);
-    g.o_z[ob + 2 * i + 1] = T(ohi[i * kWave]);
+    g.o_z[ob + int64_t(i) * kOutStride] = T(olo[i * kWave]);
+    g.o_z[ob + int64_t(i) * kOutStride + 1] = T(ohi[i * kWave]);
   }
   g.o_start[col] = ob;
   g.o_cnt[col] = 2 * n;
@@ -1858,6 +1891,110 @@ __device__ __forceinline__ void store_at(V* base, uint32_t i, V v) {
   *reinterpret_cast<V*>(reinterpret_cast<char*>(base) + i * uint32_t(sizeof(V))) = v;
 }
 
+// Where a lane's pass-1 triples go.  With kRegs (FP32 storage, R <= 31) the
+// lane's current block of kSlotBlock triples is kept in registers (a shift
+// register: the newest triple enters at the top) and written with 16-byte
+// stores when it fills, together with its summary, so every store covers
+// whole 32-byte sectors of the slot layout and the summaries need no second
+// pass.  Otherwise (no registers to spare beside the wider lane state) each
+// triple is stored as it comes and the summaries are written at the end from
+// the stored blocks.
+template <class T, bool kRegs>
+struct BlockStage {
+  static constexpr int B = kSlotBlock;
+  // the wavefront's first slot in each array and its first summary (uniform)
+  T* lo_w;
+  T* hi_w;
+  uint16_t* dy_w;
+  T* slo_w;
+  T* shi_w;
+  uint32_t L, ny;
+  int caps;
+  // slots 0-3 and 4-7 as 4-vectors, in the registers of the 16-byte stores
+  float4 la = {}, lb = {}, ha = {}, hb = {};
+  uint64_t dy = 0;  // the block's line offsets, 8 bits each (offsets <= 127)
+
+  __device__ __forceinline__ uint32_t slot(int j) const {
+    return L * B + uint32_t(j / B) * (ny * B) + uint32_t(j % B);
+  }
+  // writes stage block `blk` (its triples in slots 0 .. B - 1) and its
+  // summary (lo of the first triple, hi of the last)
+  __device__ __forceinline__ void store_block(int blk, T last) {
+    const uint32_t s = slot(blk * B);
+    float4* lw = reinterpret_cast<float4*>(reinterpret_cast<char*>(lo_w) + s * 4u);
+    float4* hw = reinterpret_cast<float4*>(reinterpret_cast<char*>(hi_w) + s * 4u);
+    lw[0] = la;
+    lw[1] = lb;
+    hw[0] = ha;
+    hw[1] = hb;
+    // 8-bit offsets widened to the 16-bit slots: bytes (b0, 0, b1, 0), ...
+    const uint32_t d0 = uint32_t(dy), d1 = uint32_t(dy >> 32);
+    const uint4 q = make_uint4(__builtin_amdgcn_perm(0u, d0, 0x0c010c00u), __builtin_amdgcn_perm(0u, d0, 0x0c030c02u),
+                               __builtin_amdgcn_perm(0u, d1, 0x0c010c00u), __builtin_amdgcn_perm(0u, d1, 0x0c030c02u));
+    *reinterpret_cast<uint4*>(reinterpret_cast<char*>(dy_w) + s * 2u) = q;
+    store_at(slo_w, L + uint32_t(blk) * ny, T(la.x));
+    store_at(shi_w, L + uint32_t(blk) * ny, last);
+  }
+  // one step of the shift register: slot i takes slot i + 1, slot 7 takes v
+  __device__ __forceinline__ static void shift_in(bool on, float4& a, float4& b, float v) {
+    a.x = on ? a.y : a.x;
+    a.y = on ? a.z : a.y;
+    a.z = on ? a.w : a.z;
+    a.w = on ? b.x : a.w;
+    b.x = on ? b.y : b.x;
+    b.y = on ? b.z : b.y;
+    b.z = on ? b.w : b.z;
+    b.w = on ? v : b.w;
+  }
+  // triple j = (a, b, d) of the lane where `put`
+  __device__ __forceinline__ void put(bool on, int j, T a, T b, int d) {
+    if constexpr (kRegs) {
+      shift_in(on, la, lb, float(a));
+      shift_in(on, ha, hb, float(b));
+      dy = on ? (dy >> 8) | (uint64_t(uint32_t(d)) << 56) : dy;
+      if (on && j % B == B - 1 && j < caps) store_block(j / B, b);
+    } else {
+      if (on && j < caps) {
+        const uint32_t s = slot(j);
+        store_at(lo_w, s, a);
+        store_at(hi_w, s, b);
+        store_at(dy_w, s, uint16_t(d));
+      }
+    }
+  }
+  // after the lane's last triple (count triples, the last with hi `last`)
+  __device__ __forceinline__ void finish(int count, T last, bool active) {
+    const int kept = min(count, caps);
+    if constexpr (kRegs) {
+      // a partial last block: its n triples sit at the top of the stage;
+      // shift them down by B - n into slots 0 .. n - 1
+      const int n = count % B;
+      const bool part = active && n > 0 && count <= caps;
+      const int sh = B - n;
+      for (int k = 0; k < B - 1; ++k) {  // at most 7 single steps, same for all lanes
+        const bool m = part && k < sh;
+        if (ballot(m) == 0) break;
+        shift_in(m, la, lb, 0.f);
+        shift_in(m, ha, hb, 0.f);
+        dy = m ? dy >> 8 : dy;
+      }
+      if (part) store_block(count / B, last);
+    } else {
+      // summaries from the stored blocks, block by block for all lanes
+      const int nblk = active ? (kept + B - 1) / B : 0;
+      const int nblk_max = wave_max(nblk);
+      for (int b = 0; b < nblk_max; ++b) {
+        if (b < nblk) {
+          const uint32_t s = slot(b * B);
+          const int lastj = min(kept - b * B, B) - 1;
+          store_at(slo_w, L + uint32_t(b) * ny, lo_w[s]);
+          store_at(shi_w, L + uint32_t(b) * ny, hi_w[s + uint32_t(lastj)]);
+        }
+      }
+    }
+  }
+};
+
 // Sweep state of one lane.  The lane's own column is window position L + R;
 // a covered window column L + k within reach (0 <= k <= 2R) is kept as one
 // bit indexed by its distance d = |k - R|, so the distance to the nearest
@@ -1879,13 +2016,6 @@ struct ZState {
   // loop-carried flags as 0/1 integers: in vector registers they need no
   // exec-mask merges at every join of the walk
   int point_open = 0, ok = 1;
-  // the wavefront's first slot in each array (uniform) and this lane's slot
-  // offset from it for triple 0
-  T* lo_w = nullptr;
-  T* hi_w = nullptr;
-  uint16_t* dy_w = nullptr;
-  uint32_t slot0 = 0;
-  uint32_t ny = 0;
   T last_hi = T(0);  // hi of the lane's last triple
   // the lane's pending group of points of equal z (their bits stay set until
   // the lane's next event that is not a point at the same z)
@@ -1941,16 +2071,11 @@ struct ZState {
 
   // close the open piece and open the next one when the distance changed;
   // zero-length pieces are kept only when a point opened them
-  __device__ __forceinline__ void emit(T z, bool is_point, bool on, bool active, const Pass1ZArgs<T>& g) {
+  template <class S>
+  __device__ __forceinline__ void emit(T z, bool is_point, bool on, bool active, const Pass1ZArgs<T>& g, S& stage) {
     const bool changed = on && dist != dcur;
     const bool put = changed && dcur <= R && (z0 < z || point_open != 0) && active;
-    if (put && count < g.caps) {
-      const uint32_t j = uint32_t(count);
-      const uint32_t s = slot0 + (j / kSlotBlock) * (ny * kSlotBlock) + j % kSlotBlock;
-      store_at(lo_w, s, z0);
-      store_at(hi_w, s, z);
-      store_at(dy_w, s, uint16_t(dcur));
-    }
+    stage.put(put, count, z0, z, dcur);
     ok = (put && count >= g.caps) ? 0 : ok;
     last_hi = put ? z : last_hi;
     count += put ? 1 : 0;
@@ -1961,23 +2086,26 @@ struct ZState {
   }
   // a finished point group: pieces are closed at its z with the points in
   // place, the points withdrawn and the piece after them opened at that z
-  __device__ __forceinline__ void close_points(bool on, bool active, const Pass1ZArgs<T>& g) {
-    emit(pz, true, on, active, g);
+  template <class S>
+  __device__ __forceinline__ void close_points(bool on, bool active, const Pass1ZArgs<T>& g, S& stage) {
+    emit(pz, true, on, active, g, stage);
 #pragma unroll
     for (int t = 0; t < NW; ++t) {
       w[t] &= on ? ~gpts[t] : ~uint64_t(0);
       gpts[t] = on ? 0 : gpts[t];
     }
     dist = nearest();
-    emit(pz, false, on, active, g);
+    emit(pz, false, on, active, g, stage);
     pend = on ? 0 : pend;
   }
   // one event of window column L + k within reach (0 <= k <= 2R) where `on`;
   // kind 0 start, 1 point, 2 end
-  __device__ __forceinline__ void apply(T z, int kind, int k, bool on, bool active, const Pass1ZArgs<T>& g) {
+  template <class S>
+  __device__ __forceinline__ void apply(T z, int kind, int k, bool on, bool active, const Pass1ZArgs<T>& g,
+                                       S& stage) {
     const bool point = kind == 1;
     const bool flush = on && pend != 0 && !(point && z == pz);
-    if (ballot(flush) != 0) close_points(flush, active, g);
+    if (ballot(flush) != 0) close_points(flush, active, g, stage);
     uint64_t bits[NW];
     column_bits(k, on, bits);
 #pragma unroll
@@ -1988,7 +2116,7 @@ struct ZState {
     dist = nearest();
     pz = (on && point) ? z : pz;
     pend = (on && point) ? 1 : pend;
-    emit(z, false, on && !point, active, g);
+    emit(z, false, on && !point, active, g, stage);
   }
 };
 
@@ -2018,18 +2146,13 @@ struct CellWalk {
   uint64_t carry[MW];
   int R;
   bool active;
-  T* lo_w;
-  T* hi_w;
-  uint16_t* dy_w;
-  uint32_t slot0, ny;
   // the lane's state: its open piece starts at z0 with distance dcur
   // (R + 1: no piece)
   T z0 = T(0);
   T last_hi = T(0);
   int dcur, count = 0, ok = 1;
 
-  __device__ __forceinline__ CellWalk(int r, bool act, T* lo, T* hi, uint16_t* dy, uint32_t s0, uint32_t n)
-      : R(r), active(act), lo_w(lo), hi_w(hi), dy_w(dy), slot0(s0), ny(n), dcur(r + 1) {
+  __device__ __forceinline__ CellWalk(int r, bool act) : R(r), active(act), dcur(r + 1) {
 #pragma unroll
     for (int t = 0; t < MW; ++t) carry[t] = 0;
   }
@@ -2038,8 +2161,8 @@ struct CellWalk {
 
   // sweep the E events (ez, ep) of one z-slab; `diff` is LDS scratch of
   // kWave + 1 words
-  template <class A>
-  __device__ void sweep(const T* ez, const uint16_t* ep, int E, uint64_t* diff, const A& g) {
+  template <class A, class S>
+  __device__ void sweep(const T* ez, const uint16_t* ep, int E, uint64_t* diff, const A& g, S& stage) {
     const int L = lane_id();
     for (int base = 0; base < E; base += kWave) {
       const int cnt = min(kWave, E - base);
@@ -2126,13 +2249,7 @@ struct CellWalk {
           dn = min(min(dp, dq), R + 1);
         }
         const bool put = on && dcur <= R && z0 < z && active;
-        if (put && count < g.caps) {
-          const uint32_t j = uint32_t(count);
-          const uint32_t s = slot0 + (j / kSlotBlock) * (ny * kSlotBlock) + j % kSlotBlock;
-          store_at(lo_w, s, z0);
-          store_at(hi_w, s, z);
-          store_at(dy_w, s, uint16_t(dcur));
-        }
+        stage.put(put, count, z0, z, dcur);
         ok = (put && count >= g.caps) ? 0 : ok;
         last_hi = put ? z : last_hi;
         count += put ? 1 : 0;
@@ -2312,13 +2429,17 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
   ZState<NW, T> st(R);
   const int64_t wslot = slot_index(g.ny, g.caps, x, y0, 0);
   const int64_t wsum = summary_index(g.ny, g.caps, x, y0, 0);
-  st.lo_w = g.m_lo + wslot;
-  st.hi_w = g.m_hi + wslot;
-  st.dy_w = g.m_dy + wslot;
-  st.slot0 = uint32_t(L * kSlotBlock);
-  st.ny = uint32_t(g.ny);
+  BlockStage<T, sizeof(T) == 4 && NW == 1> stage;
+  stage.lo_w = g.m_lo + wslot;
+  stage.hi_w = g.m_hi + wslot;
+  stage.dy_w = g.m_dy + wslot;
+  stage.slo_w = g.s_lo + wsum;
+  stage.shi_w = g.s_hi + wsum;
+  stage.L = uint32_t(L);
+  stage.ny = uint32_t(g.ny);
+  stage.caps = g.caps;
   // covered-mask words of the cell walk: the window's 64 + 2R columns
-  CellWalk<T, NW == 1 ? 2 : 3> cw(R, active, st.lo_w, st.hi_w, st.dy_w, st.slot0, st.ny);
+  CellWalk<T, NW == 1 ? 2 : 3> cw(R, active);
   int b_lo = 0;
   for (int sl = 0; sl < nslab; ++sl) {
     const int b_hi = sm.slab_end[sl];
@@ -2393,7 +2514,7 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
     // walk below.
     if constexpr (NW <= 2) {
       if (cells) {
-        cw.sweep(sm.ez, sm.ep, E, sm.colmask, g);
+        cw.sweep(sm.ez, sm.ep, E, sm.colmask, g, stage);
         b_lo = b_hi;
         continue;
       }
@@ -2425,18 +2546,18 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
           mine &= mine - 1;
           const int pk = __shfl(vp, i);
           const T z = __shfl(vz, i);
-          st.apply(z, pk & 3, (pk >> 2) - L, on, active, g);
+          st.apply(z, pk & 3, (pk >> 2) - L, on, active, g, stage);
         }
       } else {
         for (int i = 0; i < cnt; ++i) {
           const int pk = __builtin_amdgcn_readlane(vp, i);
           const int k = (pk >> 2) - L;
-          st.apply(lane_bcast(vz, i), pk & 3, k, unsigned(k) <= span, active, g);
+          st.apply(lane_bcast(vz, i), pk & 3, k, unsigned(k) <= span, active, g, stage);
         }
       }
     }
     // equal z never straddles a slab (slabs end at bin boundaries)
-    if (ballot(st.pend != 0) != 0) st.close_points(st.pend != 0, active, g);
+    if (ballot(st.pend != 0) != 0) st.close_points(st.pend != 0, active, g, stage);
     b_lo = b_hi;
   }
   if (cells) {  // the cell walk's lane state is the result
@@ -2445,35 +2566,27 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
     st.last_hi = cw.last_hi;
   }
   if (ballot(active && st.ok == 0) != 0) raise_status(g.status, kStatusArenaOverflow);
-  const int kept = min(st.count, g.caps);
   if (active) g.m_cnt[count_index(g.ny, x, y0 + L)] = st.count;
   if (active && st.count > 0) g.c_hi[count_index(g.ny, x, y0 + L)] = st.last_hi;
-  // Block summaries (lo of a block's first triple, hi of its last), written
-  // block by block for all lanes at once so that each store is one
-  // contiguous run; the values are read back from this lane's own blocks.
-  const int nblk = (kept + kSlotBlock - 1) / kSlotBlock;
-  const int nblk_max = wave_max(nblk);
-  T* const slo_w = g.s_lo + wsum;
-  T* const shi_w = g.s_hi + wsum;
-  for (int b = 0; b < nblk_max; ++b) {
-    if (b < nblk) {
-      const uint32_t s = st.slot0 + uint32_t(b) * (st.ny * kSlotBlock);
-      const int last = min(kept - b * kSlotBlock, kSlotBlock) - 1;
-      store_at(slo_w, uint32_t(L) + uint32_t(b) * st.ny, st.lo_w[s]);
-      store_at(shi_w, uint32_t(L) + uint32_t(b) * st.ny, st.hi_w[s + uint32_t(last)]);
-    }
-  }
+  stage.finish(st.count, st.last_hi, active);
+}
+
+// Waves per SIMD the z-sweep is compiled for: four (at most 128 registers;
+// the register-staged blocks take about 130 unbounded), or fewer where the
+// LDS of a CU's 160 KB allows fewer (one workgroup of one wave each).
+constexpr int zsweep_waves(size_t lds_bytes) {
+  return int(160 * 1024 / lds_bytes) / 4 < 1 ? 1 : (int(160 * 1024 / lds_bytes) / 4 > 4 ? 4 : int(160 * 1024 / lds_bytes) / 4);
 }
 
 template <class T, int ECAP, int NB, int WMAX, int NW>
-__global__ __launch_bounds__(kWave) void pass1_zsweep_kernel(Pass1ZArgs<T> g) {
+__global__ __launch_bounds__(kWave, zsweep_waves(sizeof(Pass1ZSmem<T, ECAP, NB, WMAX>))) void pass1_zsweep_kernel(Pass1ZArgs<T> g) {
   __shared__ Pass1ZSmem<T, ECAP, NB, WMAX> sm;
   zsweep_tile<T, ECAP, NB, WMAX, NW>(g, xcd_tile(blockIdx.x, gridDim.x), sm);
 }
 
 // Batched: job j owns the global tiles [tile_base[j], tile_base[j + 1]).
 template <class T, int ECAP, int NB, int WMAX, int NW>
-__global__ __launch_bounds__(kWave) void pass1_zsweep_batch_kernel(const Pass1ZArgs<T>* jobs,
+__global__ __launch_bounds__(kWave, zsweep_waves(sizeof(Pass1ZSmem<T, ECAP, NB, WMAX>))) void pass1_zsweep_batch_kernel(const Pass1ZArgs<T>* jobs,
                                                                    const int64_t* tile_base, int njobs,
                                                                    int64_t t0) {
   __shared__ Pass1ZSmem<T, ECAP, NB, WMAX> sm;
