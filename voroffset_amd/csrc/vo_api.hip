@@ -164,6 +164,10 @@ struct vo_context {
   BatchGraph bgraph;              // the last batch program, captured on its second call
   bool use_graphs = true;         // VO_BATCH_GRAPH=0: never capture
   int last_fallback_tiles = 0;
+  // the radius whose gather table the H-table slot holds, and where (a call
+  // with the same radius skips the upload)
+  double htab_R = -1;
+  const void* htab_at = nullptr;
   std::vector<void*> batch_extra;  // outputs of batched jobs redone alone (freed by the next batch)
   // host copies of the input-check tables (every call ends synchronised, so
   // they outlive their uploads)
@@ -312,12 +316,24 @@ void append_htab(double R, std::vector<double>& out) {
 // The tables of `radii`, concatenated, uploaded to `slot`.  `host` keeps the
 // values alive until the upload has run (the caller synchronises before it
 // goes out of scope).  Returns the device base.
+// A table already in place for the same single radius (the slot is written
+// by nothing else and is reallocated only to grow, that is for another
+// radius) is not uploaded again; the no-alloc scratch is shared by other
+// calls, so there the table is always uploaded.
 const double* upload_htab(vo_context* ctx, int slot, const std::vector<double>& radii, std::vector<double>& host) {
   host.clear();
   for (double R : radii) append_htab(R, host);
   double* d = static_cast<double*>(ws(ctx, slot, host.size() * sizeof(double)));
-  check(ctx, hipMemcpyAsync(d, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream),
-        "upload H table");
+  const bool cached = !ctx->arena.on && radii.size() == 1 && radii[0] == ctx->htab_R && d == ctx->htab_at;
+  if (!cached) {
+    ctx->htab_R = -1;  // (until the upload is in the stream)
+    check(ctx, hipMemcpyAsync(d, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream),
+          "upload H table");
+    if (!ctx->arena.on && radii.size() == 1) {
+      ctx->htab_R = radii[0];
+      ctx->htab_at = d;
+    }
+  }
   return d;
 }
 
