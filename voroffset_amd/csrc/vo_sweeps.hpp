@@ -807,7 +807,9 @@ __device__ void check_column(const InputCheck<T>& g, int64_t c, int job, int* wo
 
 // One volume: a wavefront checks 64 consecutive columns, whose endpoints are
 // one contiguous range, reading it with coalesced loads 64 endpoints at a
-// time; the columns starting inside each such chunk are one 64-bit mask (an
+// time, four such loads in flight together (the kernel waits on memory
+// latency, not bandwidth); the columns starting inside each chunk are one
+// 64-bit mask (an
 // OR over the lanes), so that an endpoint is compared with the one before it
 // only within its column.  For the ranged operations every endpoint is
 // checked against [lo, hi] (with the columns sorted that is the same as
@@ -827,25 +829,38 @@ __global__ __launch_bounds__(256) void check_input_kernel(InputCheck<T> g, int64
   const int64_t S = __shfl(s, 0), E = __shfl(e, last);
   if (ballot(bad != 0) == 0 && E > S) {
     T carry = T(0);
-    for (int64_t base = S; base < E; base += kWave) {
-      const int64_t i = base + L;
-      const bool in = i < E;
-      const T v = in ? g.z[i] : T(0);
-      // bit k: an endpoint base + k starts a (non-empty) column
-      const uint64_t mine = (has && s < e && s >= base && s < base + kWave) ? (uint64_t(1) << (s - base)) : 0;
-      uint64_t starts = mine;
+    // four chunks of 64 endpoints loaded together (one memory round trip)
+    constexpr int U = 4;
+    for (int64_t sup = S; sup < E; sup += U * kWave) {
+      T vs[U];
 #pragma unroll
-      for (int d = 1; d < kWave; d <<= 1) starts |= __shfl_xor(starts, d);
-      T pv = __shfl_up(v, 1);
-      pv = L == 0 ? carry : pv;
-      const bool first = ((starts >> L) & 1) != 0 || i == S;
-      int b = 0;
-      if (in) {
-        if (!isfinite(double(v)) || (!first && v < pv)) b = kInputMalformed;
-        else if (g.range && (double(v) < g.lo || double(v) > g.hi)) b = kInputOutOfRange;
+      for (int u = 0; u < U; ++u) {
+        const int64_t i = sup + u * kWave + L;
+        vs[u] = i < E ? g.z[i] : T(0);
       }
-      bad |= b;
-      carry = __shfl(v, kWave - 1);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t base = sup + u * kWave;
+        if (base >= E) break;  // uniform
+        const int64_t i = base + L;
+        const bool in = i < E;
+        const T v = vs[u];
+        // bit k: an endpoint base + k starts a (non-empty) column
+        const uint64_t mine = (has && s < e && s >= base && s < base + kWave) ? (uint64_t(1) << (s - base)) : 0;
+        uint64_t starts = mine;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) starts |= __shfl_xor(starts, d);
+        T pv = __shfl_up(v, 1);
+        pv = L == 0 ? carry : pv;
+        const bool first = ((starts >> L) & 1) != 0 || i == S;
+        int b = 0;
+        if (in) {
+          if (!isfinite(double(v)) || (!first && v < pv)) b = kInputMalformed;
+          else if (g.range && (double(v) < g.lo || double(v) > g.hi)) b = kInputOutOfRange;
+        }
+        bad |= b;
+        carry = __shfl(v, kWave - 1);
+      }
     }
   }
   if (bad) {
