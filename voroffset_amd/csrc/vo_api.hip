@@ -484,8 +484,8 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
     };
     size_arena();
     int* tiles = static_cast<int*>(ws(ctx, kTILES, (ntiles + 1) * sizeof(int)));
-    check(ctx, hipMemsetAsync(status, 0, sizeof(int), ctx->stream), "memset status");
-    check(ctx, hipMemsetAsync(tiles, 0, sizeof(int), ctx->stream), "memset tiles");
+    hipLaunchKernelGGL(vo::reset_words_kernel, dim3(1), dim3(1), 0, ctx->stream, status, tiles, o_len + ncols);
+    check(ctx, hipGetLastError(), "reset words");
     if (grid1 > INT32_MAX || ntiles > INT32_MAX || sweep_blocks > INT32_MAX)
       fail(ctx, VO_ERR_ARGUMENT, "grid too large");
 
@@ -529,15 +529,10 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
         // reads the overflow count and the status: when neither asks for
         // more work, its total is final and the dilation is done.
         check(ctx, hipEventRecord(ctx->ev[2], ctx->stream), "event");
-        check(ctx, hipMemsetAsync(o_len + ncols, 0, sizeof(int64_t), ctx->stream), "memset");
         exclusive_scan(ctx, o_len, out_off, ncols + 1);
-        check(ctx, hipMemcpyAsync(ctx->host_words + 2, tiles, sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
-              "copy word");
-        check(ctx, hipMemcpyAsync(ctx->host_words + 1, status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
-              "copy status");
-        check(ctx, hipMemcpyAsync(ctx->host_words, out_off + ncols, sizeof(int64_t), hipMemcpyDeviceToHost,
-                                  ctx->stream),
-              "copy word");
+        hipLaunchKernelGGL(vo::report_words_kernel, dim3(1), dim3(1), 0, ctx->stream, out_off + ncols, status, tiles,
+                           ctx->host_words);
+        check(ctx, hipGetLastError(), "report words");
         sync(ctx);
         std::memcpy(&nfall, ctx->host_words + 2, sizeof(int));
         if (nfall == 0 && int(ctx->host_words[1] & 0xffffffff) == 0) fast_total = ctx->host_words[0];
@@ -645,8 +640,8 @@ void enqueue_input_check(vo_context* ctx, const std::vector<vo::InputCheck<T>>& 
   const int K = int(jobs.size());
   ctx->check_jobs = job_ids;
   int* word = static_cast<int*>(ws(ctx, kV_WORD, 2 * sizeof(int)));
-  check(ctx, hipMemsetAsync(word, 0, sizeof(int), ctx->stream), "memset");
-  check(ctx, hipMemsetAsync(word + 1, 0x7f, sizeof(int), ctx->stream), "memset");  // a large job index
+  hipLaunchKernelGGL(vo::init_check_words_kernel, dim3(1), dim3(1), 0, ctx->stream, word);
+  check(ctx, hipGetLastError(), "input check words");
   ctx->check_base.assign(size_t(K) + 1, 0);
   for (int k = 0; k < K; ++k) ctx->check_base[k + 1] = ctx->check_base[k] + ncols[k];
   const int64_t total = ctx->check_base[K];
@@ -1757,7 +1752,8 @@ int vo_create(vo_context** out, int device) {
     if (device < 0 || device >= n) fail(ctx, VO_ERR_ARGUMENT, "no such HIP device");
     check(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "stream");
     for (auto& e : ctx->ev) check(ctx, hipEventCreate(&e), "event");
-    check(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->host_words), 8 * sizeof(int64_t), hipHostMallocDefault),
+    // coherent: kernels write the control words into it directly
+    check(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->host_words), 8 * sizeof(int64_t), hipHostMallocCoherent),
           "pinned");
     if (device < kMaxDevices) {  // this device's copy of the no-alloc scratch (device current: guarded)
       std::lock_guard<std::mutex> lock(g_scratch_mu);

@@ -742,6 +742,30 @@ __global__ __launch_bounds__(kWave) void pass2_kernel(Pass2Args<T> g) {
 // Column-parallel helpers (one thread per column)
 // ---------------------------------------------------------------------------
 
+// The dilation's control words cleared in one launch before pass 1: the
+// status, the count of tiles the gather gives back, and the element past the
+// output lengths (so that their exclusive scan ends with the total).
+__global__ void reset_words_kernel(int* status, int* tiles, int64_t* len_end) {
+  *status = 0;
+  *tiles = 0;
+  *len_end = 0;
+}
+// ... and reported in one launch to the host's pinned words (device-mapped
+// memory, read after the stream is synchronised): [0] the output total,
+// [1] the status, [2] the tiles given back.
+__global__ void report_words_kernel(const int64_t* total, const int* status, const int* tiles, int64_t* host) {
+  host[0] = *total;
+  host[1] = int64_t(uint32_t(*status));
+  host[2] = int64_t(uint32_t(*tiles));
+  __threadfence_system();
+}
+// The input check's words before its kernel: no failure yet, and the
+// largest job index (atomicMin keeps the smallest failing one).
+__global__ void init_check_words_kernel(int* word) {
+  word[0] = 0;
+  word[1] = 0x7f7f7f7f;
+}
+
 // Gather per-column results (start, count) into CSR at precomputed offsets.
 // A column whose start is below `interleaved` was written by the gather form
 // of pass 2: its interval i is at start + i * kOutStride (the wavefront's
@@ -1185,8 +1209,10 @@ __device__ __forceinline__ void load_block(const double* p, double (&v)[kSlotBlo
   }
 }
 
-// Source columns whose ranges the gather checks per memory round trip.
-constexpr int kGatherGroup = 4;
+// Source columns whose ranges the gather checks per memory round trip: one
+// for R <= 31 (measured fastest there: the prefetched ranges hold registers
+// the fifth wave per SIMD needs), four above (97 columns per tile at R = 48).
+constexpr int gather_group(bool small_r) { return small_r ? 1 : 4; }
 
 // Largest radius whose H table is staged in LDS ((R + 1)(R + 2) doubles,
 // 8.25 KB); above it the lanes read the table from global memory
@@ -1261,13 +1287,16 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
   int k = 0;
   double clo = INFINITY, chi = -INFINITY, below = -INFINITY, above = INFINITY;
   int dirty = 0;
+  // runs parked at the top of the lane's list area (entries ocap - 1 down to
+  // ocap - sp); the list itself uses the entries below them
+  int sp = 0;
   auto unite_slow = [&](double s, double e) {
     if (dirty) {
       olo[k * kWave] = clo;
       ohi[k * kWave] = chi;
       dirty = 0;
     }
-    if (!lane_unite(olo, ohi, ocap, n, k, s, e)) {
+    if (!lane_unite(olo, ohi, ocap - sp, n, k, s, e)) {
       ok = 0;
       return;
     }
@@ -1278,19 +1307,25 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
   };
   // unites [s, e] where `want` holds; written with selects so that the
   // wavefront branches only when some lane needs the list
-  // A run that needs the list is deferred: each lane holds at most one such
-  // run, and the held runs of all lanes go through the list together at the
-  // end of the source column (or when a lane needs a second one), so that
-  // lanes needing the list at different triples share one pass of the
-  // wavefront through that path.  The union does not depend on the order of its
-  // pieces, and a block is skipped only when it lies inside the cached
-  // interval, which is part of the union whatever is still held.
-  double hs = 0, he = 0;
-  int held = 0;
+  // A run that needs the list is parked at the top of the lane's list area
+  // (two stores), and the parked runs of all lanes go through the list
+  // together at the end of the source column, or before a block that could
+  // fill a lane's area, so that lanes needing the list at different triples
+  // share passes of the wavefront through that path and the list code
+  // appears only where runs are flushed.  The union does not depend on the
+  // order of its pieces, and a block is skipped only when it lies inside the
+  // cached interval, which is part of the union whatever is still parked.
   auto flush_held = [&]() {
-    if (ballot(held != 0) != 0) {
-      if (held != 0) unite_slow(hs, he);
-      held = 0;
+    for (;;) {
+      const bool h = sp > 0;
+      if (ballot(h) == 0) break;
+      double s = 0, e = 0;
+      if (h) {  // the most recently parked run
+        --sp;
+        s = olo[(ocap - 1 - sp) * kWave];
+        e = ohi[(ocap - 1 - sp) * kWave];
+      }
+      if (h && ok) unite_slow(s, e);
     }
   };
   auto unite = [&](bool want, double s, double e) {
@@ -1301,13 +1336,17 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
     chi = f && e > chi ? e : chi;
     dirty = f ? 1 : dirty;
     const bool slow = want && !fast;
-    if (ballot(slow && held != 0) != 0) {
-      if (slow && held != 0) unite_slow(hs, he);
-      held = (slow && held != 0) ? 0 : held;
+    if (ballot(slow) != 0) {
+      if (slow) {
+        if (n + sp < ocap) {
+          olo[(ocap - 1 - sp) * kWave] = s;
+          ohi[(ocap - 1 - sp) * kWave] = e;
+          ++sp;
+        } else {
+          ok = 0;
+        }
+      }
     }
-    hs = slow ? s : hs;
-    he = slow ? e : he;
-    held = slow ? 1 : held;
   };
   constexpr int B = kSlotBlock;
   // The lane's block of B contiguous triples of column c from jb on (jb is a
@@ -1341,9 +1380,10 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
     const uint64_t q1 = (uint64_t(bk.q.w) << 32 | bk.q.z) |
                         (nv >= 8 ? 0 : nv <= 4 ? ~uint64_t(0) : ~uint64_t(0) << (16 * (nv - 4)));
     auto hread = [&](int u) { return hrow[min(int(((u < 4 ? q0 : q1) >> (16 * (u & 3))) & 0xffffu), hsent)]; };
-    // with FP32 storage the block's table reads go first, in flight together
-    // (FP64 storage has no registers left for that)
-    constexpr bool kAhead = sizeof(T) == 4;
+    // with FP32 storage and R > 31 the block's table reads go first, in
+    // flight together (measured: no gain at R <= 31, where the registers
+    // go to a fifth wave, and FP64 storage has none left)
+    constexpr bool kAhead = sizeof(T) == 4 && !kLdsTable;
     double hv[B];
     if constexpr (kAhead) {
 #pragma unroll
@@ -1380,13 +1420,13 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
     // H(d, 0), lies inside every lane's cached interval changes nothing (the
     // union only grows) and is neither loaded nor processed.
     //
-    // Columns are checked in groups of kGatherGroup: a group's counts and
+    // Columns are checked in groups of gather_group(): a group's counts and
     // column ranges (first lo from the block summaries, last hi) are loaded
     // while the group before is processed, so the wavefront waits on memory
     // about once per group of skipped columns.  A column that is processed
     // has its count and first summary loaded again (one round trip), and the
     // next block's summary is loaded while the current block is processed.
-    constexpr int G = kGatherGroup;
+    constexpr int G = gather_group(kLdsTable);
     const int nsteps = 2 * rint + 1;
     auto col_of_step = [&](int step) { return x + ((step & 1) ? -((step + 1) >> 1) : ((step + 1) >> 1)); };
     int pm[G];
@@ -1437,6 +1477,9 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
         int jb = 0, jw = 0;
         auto drain = [&]() {
           while (ballot(want != 0) != 0) {
+            // a block parks at most B + 1 runs per lane: flush before one
+            // that could fill a lane's list area
+            if (ballot(sp > 0 && n + sp + B + 1 > ocap) != 0) flush_held();
             const bool has = want != 0;
             const int k = has ? __builtin_ctzll(want) : 0;
             want &= want - 1;
@@ -1512,21 +1555,22 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
 }
 
 // Waves per SIMD the gather is compiled for: the kernel waits on memory most
-// of the time, so occupancy pays until the registers spill (at 5 waves the
-// FP32 instantiation spills and runs slower).
-template <class T>
+// of the time, so occupancy pays until the registers spill: five for
+// R <= 31 (at most 96 registers), four above, where the prefetched column
+// ranges and table reads take more.
+template <class T, bool kLdsTable>
 struct GatherWaves {
-  static constexpr int value = 4;
+  static constexpr int value = kLdsTable ? 5 : 4;
 };
 
 template <class T, bool kLdsTable>
-__global__ __launch_bounds__(kWave, GatherWaves<T>::value) void pass2_gather_kernel(Pass2GArgs<T> g) {
+__global__ __launch_bounds__(kWave, (GatherWaves<T, kLdsTable>::value)) void pass2_gather_kernel(Pass2GArgs<T> g) {
   gather_tile<T, kLdsTable>(g, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // Batched: job j owns the global tiles [tile_base[j], tile_base[j + 1]).
 template <class T, bool kLdsTable>
-__global__ __launch_bounds__(kWave, GatherWaves<T>::value) void pass2_gather_batch_kernel(const Pass2GArgs<T>* jobs,
+__global__ __launch_bounds__(kWave, (GatherWaves<T, kLdsTable>::value)) void pass2_gather_batch_kernel(const Pass2GArgs<T>* jobs,
                                                                    const int64_t* tile_base, int njobs,
                                                                    int64_t t0) {
   const int64_t t = t0 + xcd_tile(blockIdx.x, gridDim.x);
