@@ -49,7 +49,7 @@ struct Tuning {
 };
 // pass-1 z-sweep form: LDS events per z-slab (denser windows take more slabs)
 // and histogram bins; small enough for ~5 waves per SIMD
-constexpr int kZECap = 768, kZNB = 256, kZWMax = 64 + 2 * 128;
+constexpr int kZECap = 768, kZNB = 512, kZWMax = 64 + 2 * 128;
 
 template <class T>
 struct Shapes;
@@ -353,10 +353,10 @@ void launch_zsweep(vo_context* ctx, int rint, int64_t grid, const vo::Pass1ZArgs
 template <class T>
 void launch_zsweep_events(vo_context* ctx, int events, int rint, int64_t grid, const vo::Pass1ZArgs<T>& az) {
   switch (events) {
-    case 512: launch_zsweep<T, 512, 128>(ctx, rint, grid, az); break;
-    case 1024: launch_zsweep<T, 1024, 256>(ctx, rint, grid, az); break;
+    case 512: launch_zsweep<T, 512, 256>(ctx, rint, grid, az); break;
+    case 1024: launch_zsweep<T, 1024, 512>(ctx, rint, grid, az); break;
     case 1536: launch_zsweep<T, 1536, 512>(ctx, rint, grid, az); break;
-    default: launch_zsweep<T, 768, 256>(ctx, rint, grid, az); break;
+    default: launch_zsweep<T, 768, 512>(ctx, rint, grid, az); break;
   }
 }
 
@@ -381,10 +381,10 @@ template <class T>
 void launch_zsweep_batch_events(vo_context* ctx, int events, int max_rint, int64_t grid,
                                 const vo::Pass1ZArgs<T>* jobs, const int64_t* tile_base, int njobs, int64_t t0) {
   switch (events) {
-    case 512: launch_zsweep_batch<T, 512, 128>(ctx, max_rint, grid, jobs, tile_base, njobs, t0); break;
-    case 1024: launch_zsweep_batch<T, 1024, 256>(ctx, max_rint, grid, jobs, tile_base, njobs, t0); break;
+    case 512: launch_zsweep_batch<T, 512, 256>(ctx, max_rint, grid, jobs, tile_base, njobs, t0); break;
+    case 1024: launch_zsweep_batch<T, 1024, 512>(ctx, max_rint, grid, jobs, tile_base, njobs, t0); break;
     case 1536: launch_zsweep_batch<T, 1536, 512>(ctx, max_rint, grid, jobs, tile_base, njobs, t0); break;
-    default: launch_zsweep_batch<T, 768, 256>(ctx, max_rint, grid, jobs, tile_base, njobs, t0); break;
+    default: launch_zsweep_batch<T, 768, 512>(ctx, max_rint, grid, jobs, tile_base, njobs, t0); break;
   }
 }
 

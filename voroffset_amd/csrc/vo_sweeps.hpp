@@ -2356,6 +2356,7 @@ __device__ __forceinline__ uint64_t lane_event_mask(uint64_t* colmask, int col, 
 
 template <class T, int ECAP, int NB, int WMAX, int NW>
 __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, ECAP, NB, WMAX>& sm) {
+  static_assert(NB <= 32 * kWave, "one bit per lane's bin chunk in a 32-bit mask");
   const int L = lane_id();
   const int x = int(tile % g.nx);
   const int y0 = int(tile / g.nx) * kZTile;
@@ -2503,12 +2504,15 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
   for (int sl = 0; sl < nslab; ++sl) {
     const int b_hi = sm.slab_end[sl];
     wave_sync();
-    // exclusive bin starts of this slab
+    // exclusive bin starts of this slab; bit t of `unsorted`: the lane's bin
+    // b_lo + L + 64 t holds two events or more
+    uint32_t unsorted = 0;
     {
       int carry = 0;
       for (int base = b_lo; base < b_hi; base += kWave) {
         const int b = base + L;
         const int c = b < b_hi ? sm.bin[b] : 0;
+        unsorted |= c >= 2 ? 1u << ((base - b_lo) / kWave) : 0u;
         int v = c;
 #pragma unroll
         for (int d = 1; d < kWave; d <<= 1) {
@@ -2551,8 +2555,10 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
       });
     }
     wave_sync();
-    // every bin now holds its end offset: order each bin by (z, kind)
-    for (int b = b_lo + L; b < b_hi; b += kWave) {
+    // every bin now holds its end offset: order each bin of two events or
+    // more by (z, kind)
+    for (; unsorted != 0; unsorted &= unsorted - 1) {
+      const int b = b_lo + L + kWave * int(__builtin_ctz(unsorted));
       const int e0 = (b == b_lo) ? 0 : sm.bin[b - 1];
       const int e1 = sm.bin[b];
       for (int i = e0 + 1; i < e1; ++i) {
