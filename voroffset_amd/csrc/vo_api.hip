@@ -168,6 +168,14 @@ struct vo_context {
   // with the same radius skips the upload)
   double htab_R = -1;
   const void* htab_at = nullptr;
+  int64_t spec_out = 0;  // endpoints the next dilation's output is first sized for
+  // an input check left to the pass 1 of the operation's first dilation
+  // (word == null: none pending); see defer_input_check
+  struct {
+    int* word = nullptr;
+    int range = 0;
+    double lo = 0, hi = 0;
+  } fused;
   std::vector<void*> batch_extra;  // outputs of batched jobs redone alone (freed by the next batch)
   // host copies of the input-check tables (every call ends synchronised, so
   // they outlive their uploads)
@@ -423,6 +431,10 @@ Axis axis_for(const vo_context* ctx, int nx, int ny) {
   return Axis{nx, ny, 1, int64_t(nx)};
 }
 
+template <class T>
+void enqueue_input_check(vo_context* ctx, const std::vector<vo::InputCheck<T>>& jobs,
+                         const std::vector<int64_t>& ncols, const std::vector<int>& job_ids);
+
 // One dilation: `in` -> CSR in (out_off_slot, out_z_slot).
 template <class T>
 DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int out_z_slot, double* t1,
@@ -441,6 +453,13 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
   int64_t* o_start = static_cast<int64_t*>(ws(ctx, kO_START, ncols * sizeof(int64_t)));
   int64_t* o_len = static_cast<int64_t*>(ws(ctx, kO_LEN, (ncols + 1) * sizeof(int64_t)));
   int64_t* out_off = static_cast<int64_t*>(ws(ctx, out_off_slot, (ncols + 1) * sizeof(int64_t)));
+  // a pending input check of `in` (this is the operation's first dilation):
+  // pass 1 does it when the z-sweep form runs, else its own kernel
+  int* chk_word = ctx->fused.word;
+  const int chk_range = ctx->fused.range;
+  const double chk_lo = ctx->fused.lo, chk_hi = ctx->fused.hi;
+  ctx->fused.word = nullptr;
+  bool chk_init = chk_word != nullptr;  // the word is cleared by the first pass 1 that checks
   DevVol out;
   out.nx = in.nx;
   out.ny = in.ny;
@@ -456,6 +475,7 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
   bool zsweep_failed = false;  // a window too dense for the z-sweep form: sweep along y instead
   int64_t fast_total = -1;     // the output size, when the gather's one synchronisation settled it
   int64_t gather_end = 0;      // the gather form's part of the output arena (interleaved columns)
+  bool copied = false;         // the output CSR was copied before the gather's synchronisation
   for (int attempt = 0;; ++attempt) {
     const int nch1 = (ny + chunk1 - 1) / chunk1;
     const int64_t grid1 = int64_t(nx) * nch1;
@@ -484,19 +504,34 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
     };
     size_arena();
     int* tiles = static_cast<int*>(ws(ctx, kTILES, (ntiles + 1) * sizeof(int)));
-    hipLaunchKernelGGL(vo::reset_words_kernel, dim3(1), dim3(1), 0, ctx->stream, status, tiles, o_len + ncols);
-    check(ctx, hipGetLastError(), "reset words");
     if (grid1 > INT32_MAX || ntiles > INT32_MAX || sweep_blocks > INT32_MAX)
       fail(ctx, VO_ERR_ARGUMENT, "grid too large");
+    const bool zsweep = ctx->pass1_form == 0 && rint <= 127 && !zsweep_failed &&
+                        vo::zsweep_offsets_fit(caps, ny, int(sizeof(T)));
+    if (chk_init && !zsweep) {  // the check's own kernel, words and copy
+      vo::InputCheck<T> c{in.off, static_cast<const T*>(in.z), chk_range, chk_lo, chk_hi};
+      enqueue_input_check<T>(ctx, {c}, {ncols}, {0});
+      chk_word = nullptr;
+      chk_init = false;
+    }
+    const bool chk_here = chk_init && zsweep;
+    hipLaunchKernelGGL(vo::reset_words_kernel, dim3(1), dim3(1), 0, ctx->stream, status, tiles, o_len + ncols,
+                       chk_here ? chk_word : nullptr);
+    check(ctx, hipGetLastError(), "reset words");
 
     vo::Pass1Args<T> a1{nx, ny, rint, chunk1, nch1, in.off, static_cast<const T*>(in.z), m_lo, m_hi, m_dy,
                         caps, m_cnt, status, ax.sx, ax.sy};
     check(ctx, hipEventRecord(ctx->ev[0], ctx->stream), "event");
-    const bool zsweep = ctx->pass1_form == 0 && rint <= 127 && !zsweep_failed &&
-                        vo::zsweep_offsets_fit(caps, ny, int(sizeof(T)));
     if (zsweep) {
       vo::Pass1ZArgs<T> az{nx, ny, rint, in.off, static_cast<const T*>(in.z), m_lo, m_hi, m_dy, caps, m_cnt,
                            status, s_lo, s_hi, c_hi, ax.sx, ax.sy};
+      if (chk_here) {
+        az.check_word = chk_word;
+        az.check_range = chk_range;
+        az.check_lo = chk_lo;
+        az.check_hi = chk_hi;
+        chk_init = false;
+      }
       const int64_t gridz = int64_t(nx) * ((ny + vo::kZTile - 1) / vo::kZTile);
       launch_zsweep_events<T>(ctx, events_for(ctx, rint), rint, gridz, az);
     } else if (ctx->pass1_form == 2) {
@@ -531,11 +566,27 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
         check(ctx, hipEventRecord(ctx->ev[2], ctx->stream), "event");
         exclusive_scan(ctx, o_len, out_off, ncols + 1);
         hipLaunchKernelGGL(vo::report_words_kernel, dim3(1), dim3(1), 0, ctx->stream, out_off + ncols, status, tiles,
-                           ctx->host_words);
+                           chk_word, ctx->host_words);
         check(ctx, hipGetLastError(), "report words");
+        // The CSR copy goes before the synchronisation too, into an output
+        // sized like the last result plus a quarter (columns that would not
+        // fit are skipped): when the result fits, the dilation is complete at
+        // the synchronisation.  (Not with the no-alloc scratch, whose output
+        // buffer is the caller's.)
+        int64_t spec = 0;
+        if (!ctx->arena.on) {
+          spec = std::max<int64_t>(ctx->spec_out, 16);
+          T* oz = static_cast<T*>(ws(ctx, out_z_slot, size_t(spec) * sizeof(T)));
+          hipLaunchKernelGGL(vo::gather_kernel<T>, dim3(blocks_for(ncols, 256)), dim3(256), 0, ctx->stream, ncols,
+                             static_cast<const T*>(o_z), o_start, o_len, out_off, oz, ntiles * garena, spec);
+          check(ctx, hipGetLastError(), "gather launch");
+        }
         sync(ctx);
         std::memcpy(&nfall, ctx->host_words + 2, sizeof(int));
-        if (nfall == 0 && int(ctx->host_words[1] & 0xffffffff) == 0) fast_total = ctx->host_words[0];
+        if (nfall == 0 && int(ctx->host_words[1] & 0xffffffff) == 0) {
+          fast_total = ctx->host_words[0];
+          copied = fast_total <= spec;
+        }
         if (nfall == 0 || gocap >= 64) break;
         // lists outgrew their capacity: the whole gather again with twice the
         // room is cheaper and more robust than the sweep form for many tiles
@@ -570,6 +621,9 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
     check(ctx, hipMemsetAsync(o_len + ncols, 0, sizeof(int64_t), ctx->stream), "memset");
     check(ctx, hipMemcpyAsync(ctx->host_words + 1, status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
           "copy status");
+    if (chk_word)
+      check(ctx, hipMemcpyAsync(ctx->host_words + 3, chk_word, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
+            "copy input check");
     sync(ctx);
     const int st = int(ctx->host_words[1] & 0xffffffff);
     if (st == 0) {
@@ -603,9 +657,13 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
     total = read_word(ctx, out_off + ncols);
   }
   T* oz = static_cast<T*>(ws(ctx, out_z_slot, size_t(std::max<int64_t>(total, 1)) * sizeof(T)));
-  hipLaunchKernelGGL(vo::gather_kernel<T>, dim3(blocks_for(ncols, 256)), dim3(256), 0, ctx->stream, ncols,
-                     static_cast<const T*>(slot_ptr(ctx, kO_Z)), o_start, o_len, out_off, oz, gather_end);
-  check(ctx, hipGetLastError(), "gather launch");
+  if (!copied) {
+    hipLaunchKernelGGL(vo::gather_kernel<T>, dim3(blocks_for(ncols, 256)), dim3(256), 0, ctx->stream, ncols,
+                       static_cast<const T*>(slot_ptr(ctx, kO_Z)), o_start, o_len, out_off, oz, gather_end,
+                       std::max<int64_t>(total, 0));
+    check(ctx, hipGetLastError(), "gather launch");
+  }
+  if (!ctx->arena.on) ctx->spec_out = total + total / 4;
   out.z = oz;
   out.count = total;
   ctx->last_outputs = total;
@@ -669,6 +727,32 @@ void enqueue_input_check(vo_context* ctx, const std::vector<vo::InputCheck<T>>& 
   check(ctx, hipMemcpyAsync(ctx->host_words + 3, word, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
         "copy input check");
   ctx->check_input = true;
+}
+
+// The input check of an operation whose first step is a dilation of the
+// input (dilation, closing): left to that dilation's pass 1, which reads
+// every input column anyway (or to the check's own kernel when another pass-1
+// form runs).  Until the dilation reports the words the host copy says "no
+// failure", like the check kernel's words before its copy arrives.
+template <class T>
+void defer_input_check(vo_context* ctx, const vo::InputCheck<T>& c) {
+  ctx->fused.word = static_cast<int*>(ws(ctx, kV_WORD, 2 * sizeof(int)));
+  ctx->fused.range = c.range;
+  ctx->fused.lo = c.lo;
+  ctx->fused.hi = c.hi;
+  ctx->check_jobs = {0};
+  const int w[2] = {0, 0x7f7f7f7f};
+  std::memcpy(ctx->host_words + 3, w, sizeof(w));
+  ctx->check_input = true;
+}
+
+template <class T>
+void check_input_of_op(vo_context* ctx, int op, double R, const vo_volume& v) {
+  const vo::InputCheck<T> c = input_check_of<T>(op, R, v);
+  if (op == VO_OP_DILATION || op == VO_OP_CLOSING)
+    defer_input_check<T>(ctx, c);
+  else
+    enqueue_input_check<T>(ctx, {c}, {int64_t(v.nx) * v.ny}, {0});
 }
 
 // Per-column complement kernels (phase 0: lengths, phase 1: data): the ring
@@ -1142,7 +1226,8 @@ void morph_batch_device(vo_context* ctx, int n, const vo_job* jobs, vo_volume* o
         exclusive_scan(ctx, len, st.off, st.C + 1);
         if (st.C > 0) {  // (a step over empty grids only has its one offset)
           hipLaunchKernelGGL(vo::gather_kernel<T>, dim3(blocks_for(st.C, 256)), dim3(256), 0, ctx->stream, st.C,
-                             static_cast<const T*>(o_z), o_start, len, st.off, static_cast<T*>(st.z), INT64_MAX);
+                             static_cast<const T*>(o_z), o_start, len, st.off, static_cast<T*>(st.z), INT64_MAX,
+                             INT64_MAX);
           check(ctx, hipGetLastError(), "gather launch");
         }
       } else {
@@ -1470,18 +1555,23 @@ void autotune(vo_context* ctx, const vo_volume* sample, int n, const double* rad
 
 int guarded(vo_context* ctx, const std::function<void()>& body) {
   if (!ctx) return VO_ERR_ARGUMENT;
+  // a call's input check never outlives it
   ctx->check_input = false;
+  ctx->fused.word = nullptr;
   (void)hipGetLastError();  // an error left by an earlier, already reported call is not this call's
   try {
     check(ctx, hipSetDevice(ctx->device), "hipSetDevice");
     body();
     ctx->check_input = false;
+    ctx->fused.word = nullptr;
     return VO_OK;
   } catch (const Failure& f) {
     ctx->check_input = false;
+    ctx->fused.word = nullptr;
     return f.code;
   } catch (const std::exception& e) {
     ctx->check_input = false;
+    ctx->fused.word = nullptr;
     ctx->error = e.what();
     return VO_ERR_ARGUMENT;
   }
@@ -1719,7 +1809,7 @@ void morph_noalloc(vo_context* ctx, int op, double R, const vo_volume* in, int64
   v.ny = in->ny;
   v.off = in->offsets;
   v.z = in->z;
-  enqueue_input_check<T>(ctx, {input_check_of<T>(op, R, *in)}, {n}, {0});
+  check_input_of_op<T>(ctx, op, R, *in);
   DevVol r = run_op<T>(ctx, op, v, R, in->z_lo, in->z_hi, t1, t2);
   if (r.off != out_offsets || (r.count > 0 && r.z != out_z))
     fail(ctx, VO_ERR_DEVICE, "no-alloc: the result is not in the caller's buffers");
@@ -1928,11 +2018,11 @@ int vo_morph_device(vo_context* ctx, int op, int precision, double radius, const
       r.count = n ? read_word(ctx, in->offsets + n) : 0;
     } else if (precision == VO_FP32) {
       if (op < VO_OP_NOOP || op > VO_OP_OPENING) fail(ctx, VO_ERR_ARGUMENT, "unknown operation");
-      enqueue_input_check<float>(ctx, {input_check_of<float>(op, radius, *in)}, {int64_t(in->nx) * in->ny}, {0});
+      check_input_of_op<float>(ctx, op, radius, *in);
       r = run_op<float>(ctx, op, v, radius, in->z_lo, in->z_hi, time_1_ms, time_2_ms);
     } else if (precision == VO_FP64) {
       if (op < VO_OP_NOOP || op > VO_OP_OPENING) fail(ctx, VO_ERR_ARGUMENT, "unknown operation");
-      enqueue_input_check<double>(ctx, {input_check_of<double>(op, radius, *in)}, {int64_t(in->nx) * in->ny}, {0});
+      check_input_of_op<double>(ctx, op, radius, *in);
       r = run_op<double>(ctx, op, v, radius, in->z_lo, in->z_hi, time_1_ms, time_2_ms);
     } else {
       fail(ctx, VO_ERR_ARGUMENT, "unknown precision");

@@ -745,18 +745,25 @@ __global__ __launch_bounds__(kWave) void pass2_kernel(Pass2Args<T> g) {
 // The dilation's control words cleared in one launch before pass 1: the
 // status, the count of tiles the gather gives back, and the element past the
 // output lengths (so that their exclusive scan ends with the total).
-__global__ void reset_words_kernel(int* status, int* tiles, int64_t* len_end) {
+__global__ void reset_words_kernel(int* status, int* tiles, int64_t* len_end, int* check_word) {
   *status = 0;
   *tiles = 0;
   *len_end = 0;
+  if (check_word) {  // the input check pass 1 does (see init_check_words_kernel)
+    check_word[0] = 0;
+    check_word[1] = 0x7f7f7f7f;
+  }
 }
 // ... and reported in one launch to the host's pinned words (device-mapped
 // memory, read after the stream is synchronised): [0] the output total,
-// [1] the status, [2] the tiles given back.
-__global__ void report_words_kernel(const int64_t* total, const int* status, const int* tiles, int64_t* host) {
+// [1] the status, [2] the tiles given back, [3] the input check's two words
+// when pass 1 did the check.
+__global__ void report_words_kernel(const int64_t* total, const int* status, const int* tiles, const int* check_word,
+                                    int64_t* host) {
   host[0] = *total;
   host[1] = int64_t(uint32_t(*status));
   host[2] = int64_t(uint32_t(*tiles));
+  if (check_word) host[3] = int64_t(uint64_t(uint32_t(check_word[0])) | uint64_t(uint32_t(check_word[1])) << 32);
   __threadfence_system();
 }
 // The input check's words before its kernel: no failure yet, and the
@@ -771,13 +778,15 @@ __global__ void init_check_words_kernel(int* word) {
 // of pass 2: its interval i is at start + i * kOutStride (the wavefront's
 // i-th intervals side by side); the others are contiguous.
 constexpr int kOutStride = 2 * kWave;
+// Endpoints past `cap` (the destination's length) are not written.
 template <class T>
 __global__ void gather_kernel(int64_t ncols, const T* src, const int64_t* start, const int64_t* cnt,
-                              const int64_t* off, T* dst, int64_t interleaved) {
+                              const int64_t* off, T* dst, int64_t interleaved, int64_t cap) {
   const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (c >= ncols) return;
   const int64_t s = start[c], o = off[c];
   const int64_t n = cnt[c];
+  if (o + n > cap) return;
   if (s < interleaved) {
     for (int64_t i = 0; i < n; i += 2) {
       dst[o + i] = src[s + (i >> 1) * kOutStride];
@@ -1888,6 +1897,12 @@ struct Pass1ZArgs {
   // x * sx + y * sy (sx = 1, sy = nx sweeps along y, the reference's axis;
   // sx = ny, sy = 1 with nx, ny swapped sweeps along x)
   int64_t sx, sy;
+  // The input check of the operation, done here instead of by its own
+  // kernel (null: none): each tile checks its 64 output columns' input as
+  // the window is read, with the rules and words of check_input_kernel.
+  int* check_word = nullptr;
+  int check_range = 0;
+  double check_lo = 0, check_hi = 0;
 };
 
 constexpr int kZTile = kWave;    // output columns per wavefront
@@ -1921,11 +1936,14 @@ struct Pass1ZSmem {
 // neighbours fused, points kept only when isolated).
 // Two segments are loaded per step, so that a column costs half as many
 // memory round trips.
-template <class T, class F>
-__device__ __forceinline__ void column_segments(const T* z, int64_t s, int64_t n, F&& f) {
+// `raw` sees every stored pair (a, b) in order before it is merged.
+template <class T, class F, class P>
+__device__ __forceinline__ void column_segments(const T* z, int64_t s, int64_t n, F&& f, P&& raw) {
   if (n < 2) return;
   T ca = z[s], cb = z[s + 1];
+  raw(ca, cb);
   auto step = [&](T a, T b) {
+    raw(a, b);
     if (a <= cb) {
       if (b > cb) cb = b;
     } else {
@@ -1942,6 +1960,10 @@ __device__ __forceinline__ void column_segments(const T* z, int64_t s, int64_t n
   }
   if (k + 1 < s + n) step(z[k], z[k + 1]);
   f(ca, cb);
+}
+template <class T, class F>
+__device__ __forceinline__ void column_segments(const T* z, int64_t s, int64_t n, F&& f) {
+  column_segments(z, s, n, f, [](T, T) {});
 }
 
 // base[i] as a scalar base plus a 32-bit vector byte offset
@@ -2371,6 +2393,32 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
     s = g.in_off[c];
     n = g.in_off[c + 1] - s;
   };
+  // the input check of the tile's own columns (window positions R .. R + 63):
+  // offsets here, endpoints as the histogram pass reads them
+  const bool chk = g.check_word != nullptr;
+  int bad = 0;
+  if (chk && y0 + L < g.ny) {
+    const int64_t c = int64_t(x) * g.sx + int64_t(y0 + L) * g.sy;
+    const int64_t s = g.in_off[c], e = g.in_off[c + 1];
+    bad = (e < s || ((e - s) & 1)) ? kInputMalformed : 0;
+  }
+  auto raw_check = [&](bool own, T& prev) {
+    return [&, own](T a, T b) {
+      // in order and finite (every comparison with a NaN fails)
+      const bool okv = a >= prev && b >= a && fabs(a) < T(INFINITY) && fabs(b) < T(INFINITY);
+      int m = own && !okv ? kInputMalformed : 0;
+      if (g.check_range)  // uniform
+        m = m ? m : ((own && (double(a) < g.check_lo || double(b) > g.check_hi)) ? kInputOutOfRange : 0);
+      bad |= m;
+      prev = b;
+    };
+  };
+  auto report_check = [&]() {
+    if (chk && bad) {
+      atomicOr(g.check_word, bad);
+      atomicMin(g.check_word + 1, 0);
+    }
+  };
   // z-range of the window
   double zlo = INFINITY, zhi = -INFINITY;
   for (int w = L; w < wn; w += kWave) {
@@ -2387,8 +2435,15 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
     zhi = fmax(zhi, __shfl_xor(zhi, d));
   }
   const bool active = y0 + L < g.ny;
-  if (!(zlo <= zhi)) {  // no input anywhere in the window
+  if (!(zlo <= zhi)) {  // no input anywhere in the window (or none that is finite)
     if (active) g.m_cnt[count_index(g.ny, x, y0 + L)] = 0;
+    if (chk && active) {
+      int64_t s, n;
+      col_range(L + R, s, n);
+      T prev = T(-INFINITY);
+      for (int64_t i = 0; i + 1 < n; i += 2) raw_check(true, prev)(g.in_z[s + i], g.in_z[s + i + 1]);
+    }
+    report_check();
     return;
   }
   const double scale = double(NB) / fmax(zhi - zlo, 1e-30) * (1.0 - 1e-12);
@@ -2400,12 +2455,17 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
   for (int w = L; w < wn; w += kWave) {
     int64_t s, n;
     col_range(w, s, n);
-    column_segments(g.in_z, s, n, [&](T a, T b) {
-      atomicAdd(&sm.bin[bin_of(a)], 1);
-      if (a < b) atomicAdd(&sm.bin[bin_of(b)], 1);
-      points |= a < b ? 0 : 1;
-    });
+    T prev = T(-INFINITY);
+    column_segments(
+        g.in_z, s, n,
+        [&](T a, T b) {
+          atomicAdd(&sm.bin[bin_of(a)], 1);
+          if (a < b) atomicAdd(&sm.bin[bin_of(b)], 1);
+          points |= a < b ? 0 : 1;
+        },
+        raw_check(chk && w >= R && w < R + kZTile && y0 + (w - R) < g.ny, prev));
   }
+  report_check();
   // windows without zero-length segments take the cell walk (R <= 63; the
   // lane state for R <= 31 has room for 128 window columns)
   const bool cells = NW <= 2 && ballot(points != 0) == 0 && R <= (NW == 1 ? 31 : kCellMaxR);
