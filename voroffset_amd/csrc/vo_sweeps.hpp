@@ -1495,25 +1495,61 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
             process_block(c, jw + k * B, m, hrow, rs, re, has);
           }
         };
-        T blo = g.s_lo[summary_index(g.ny, g.caps, c, yc, 0)];
-        T bhi = g.s_hi[summary_index(g.ny, g.caps, c, yc, 0)];
-        for (;;) {
-          const bool more = ballot(m > jb + B && ok) != 0;
-          T nblo = blo, nbhi = bhi;
-          if (more) {
-            nblo = g.s_lo[summary_index(g.ny, g.caps, c, yc, jb + B)];
-            nbhi = g.s_hi[summary_index(g.ny, g.caps, c, yc, jb + B)];
+        // Two ways to mark a column's blocks.  Forward: every block in z
+        // order, the next block's summary loaded while the current one is
+        // checked.  Two-ended: a column's block ranges increase along it (its
+        // triples are z-sorted and disjoint), so the blocks not inside the
+        // cached interval are a prefix (lo below it) and a suffix (hi above
+        // it); each lane scans from both ends to its first block inside, one
+        // summary from each end per memory round trip, and the blocks between
+        // are inside (fewer round trips; measured faster everywhere but for
+        // FP32 storage at R <= 31, where its registers cost the fifth wave).
+        constexpr bool kTwoEnded = !(kLdsTable && sizeof(T) == 4);
+        const int nb = ok ? (m + B - 1) / B : 0;  // the lane's blocks in this column
+        if (kTwoEnded && ballot(nb > 64) == 0) {
+          int f = 0, bk = nb - 1;
+          int fd = nb == 0, bd = nb == 0;
+          while (ballot(fd == 0 || bd == 0) != 0) {
+            T flo = T(0), fhi = T(0), klo = T(0), khi = T(0);
+            if (fd == 0) {
+              flo = g.s_lo[summary_index(g.ny, g.caps, c, yc, f * B)];
+              fhi = g.s_hi[summary_index(g.ny, g.caps, c, yc, f * B)];
+            }
+            if (bd == 0) {
+              klo = g.s_lo[summary_index(g.ny, g.caps, c, yc, bk * B)];
+              khi = g.s_hi[summary_index(g.ny, g.caps, c, yc, bk * B)];
+            }
+            const bool fin = double(flo) - hmax >= clo && double(fhi) + hmax <= chi;
+            const bool kin = double(klo) - hmax >= clo && double(khi) + hmax <= chi;
+            want |= (fd == 0 && !fin) ? uint64_t(1) << f : 0;
+            want |= (bd == 0 && !kin) ? uint64_t(1) << bk : 0;
+            f += (fd == 0 && !fin) ? 1 : 0;
+            bk -= (bd == 0 && !kin) ? 1 : 0;
+            fd = (fd != 0 || fin || f > bk) ? 1 : 0;
+            bd = (bd != 0 || kin || f > bk) ? 1 : 0;
           }
-          const bool inside = jb >= m || ok == 0 || (double(blo) - hmax >= clo && double(bhi) + hmax <= chi);
-          want |= inside ? 0 : uint64_t(1) << ((jb - jw) / B);
-          if (!more || jb - jw == 63 * B) {
-            drain();
-            jw = jb + B;
+          drain();
+        } else {
+          T blo = g.s_lo[summary_index(g.ny, g.caps, c, yc, 0)];
+          T bhi = g.s_hi[summary_index(g.ny, g.caps, c, yc, 0)];
+          for (;;) {
+            const bool more = ballot(m > jb + B && ok) != 0;
+            T nblo = blo, nbhi = bhi;
+            if (more) {
+              nblo = g.s_lo[summary_index(g.ny, g.caps, c, yc, jb + B)];
+              nbhi = g.s_hi[summary_index(g.ny, g.caps, c, yc, jb + B)];
+            }
+            const bool inside = jb >= m || ok == 0 || (double(blo) - hmax >= clo && double(bhi) + hmax <= chi);
+            want |= inside ? 0 : uint64_t(1) << ((jb - jw) / B);
+            if (!more || jb - jw == 63 * B) {
+              drain();
+              jw = jb + B;
+            }
+            if (!more) break;
+            jb += B;
+            blo = nblo;
+            bhi = nbhi;
           }
-          if (!more) break;
-          jb += B;
-          blo = nblo;
-          bhi = nbhi;
         }
         unite(rs == rs && ok, rs, re);
         flush_held();
@@ -1564,12 +1600,13 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
 }
 
 // Waves per SIMD the gather is compiled for: the kernel waits on memory most
-// of the time, so occupancy pays until the registers spill: five for
-// R <= 31 (at most 96 registers), four above, where the prefetched column
-// ranges and table reads take more.
+// of the time, so occupancy pays until the registers spill: five for FP32
+// storage at R <= 31 (at most 96 registers), four otherwise, where the
+// prefetched column ranges, table reads, FP64 blocks and the two-ended block
+// scan take more.
 template <class T, bool kLdsTable>
 struct GatherWaves {
-  static constexpr int value = kLdsTable ? 5 : 4;
+  static constexpr int value = kLdsTable && sizeof(T) == 4 ? 5 : 4;
 };
 
 template <class T, bool kLdsTable>
