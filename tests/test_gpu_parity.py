@@ -114,6 +114,29 @@ def test_edge_cases_fp64(hip_fp64, name, r):
     check_fp64((res.off, res.z), oracle.morph(vol, "dilation", r))
 
 
+def _dense_column_volume():
+    # one column of 600 segments (length 0.5, gaps 1.0): more than 64 blocks of
+    # pass-1 triples in that column, the case the block scans take in windows
+    # or without skipping
+    cols = [[] for _ in range(25)]
+    cols[12] = [v for k in range(600) for v in (1.5 * k, 1.5 * k + 0.5)]
+    return _vol(5, 5, cols, zext=900.0)
+
+
+@pytest.mark.parametrize("r", [0.3, 40.0])
+def test_dense_column_fp64(hip_fp64, r):
+    vol = _dense_column_volume()
+    res, _, _ = hip_fp64.dilation(vol, r)
+    check_fp64((res.off, res.z), oracle.morph(vol, "dilation", r))
+
+
+@pytest.mark.parametrize("r", [0.3, 40.0])
+def test_dense_column_fp32(hip_fp32, r):
+    vol = quantize_fp32(_dense_column_volume())
+    res, _, _ = hip_fp32.dilation(vol, r)
+    check_fp32((res.off, res.z), oracle.morph(vol, "dilation", r))
+
+
 def test_empty_grid(hip_fp64):
     vol = _vol(0, 0, [])
     res, _, _ = hip_fp64.dilation(vol, 2.0)

@@ -35,6 +35,9 @@ namespace {
 // Kernel shapes.  The LDS footprint of each instantiation is checked below.
 constexpr int kP1Cap = 128, kP1NCap = 64;
 constexpr int kP2SCap = 128, kP2PCap = 256, kP2NCap = 64, kP2OCap = 256;
+// Most workspace the gather's interval lists and output arena may take when
+// their capacity grows for tiles with many output intervals per column.
+constexpr size_t kGatherRoomBytes = size_t(4) << 30;
 // gather form: intervals per output column list.  Large radii fuse the
 // dilated pieces into few intervals, so they get the short list (and more
 // waves per CU); a tile that overflows is recomputed by the sweep form.
@@ -587,9 +590,14 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
           fast_total = ctx->host_words[0];
           copied = fast_total <= spec;
         }
-        if (nfall == 0 || gocap >= 64) break;
         // lists outgrew their capacity: the whole gather again with twice the
-        // room is cheaper and more robust than the sweep form for many tiles
+        // room is cheaper and more robust than the sweep form for many tiles,
+        // as long as the lists and the output arena of all tiles stay within
+        // kGatherRoomBytes (a column with hundreds of output intervals, on a
+        // small grid, gets the room; a large grid keeps the sweep form for
+        // its few dense tiles)
+        const size_t room = size_t(ntiles) * vo::kWave * 2 * size_t(2 * gocap) * (sizeof(double) + sizeof(T));
+        if (nfall == 0 || room > kGatherRoomBytes) break;
         gocap *= 2;
         size_arena();
         check(ctx, hipMemsetAsync(tiles, 0, sizeof(int), ctx->stream), "memset tiles");

@@ -786,7 +786,7 @@ __global__ void gather_kernel(int64_t ncols, const T* src, const int64_t* start,
   if (c >= ncols) return;
   const int64_t s = start[c], o = off[c];
   const int64_t n = cnt[c];
-  if (o + n > cap) return;
+  if (o < 0 || n < 0 || o + n > cap) return;
   if (s < interleaved) {
     for (int64_t i = 0; i < n; i += 2) {
       dst[o + i] = src[s + (i >> 1) * kOutStride];
@@ -1265,10 +1265,18 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
   const int x = int(tile % g.nx);
   const int y0 = int(tile / g.nx) * kGatherTile;
   if (y0 >= g.ny) return;
-  if (*g.status != 0) return;  // pass 1 did not complete
   const int rint = g.rint;
   const int y = y0 + L;
   const bool active = y < g.ny;
+  const int64_t col = int64_t(x) * g.sx + int64_t(y) * g.sy;
+  // every column's output length is valid even where this tile gives up (an
+  // overflow, or pass 1 incomplete): the scan and the CSR copy that follow
+  // before the host looks at the status then read nothing for it
+  if (active) {
+    g.o_start[col] = 0;
+    g.o_cnt[col] = 0;
+  }
+  if (*g.status != 0) return;  // pass 1 did not complete
   const double* htab = g.htab;
   if constexpr (kLdsTable) {
     for (int i = L; i < (rint + 1) * htab_width(rint); i += kWave) vo_htab_lds[i] = g.htab[i];
@@ -1587,7 +1595,6 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
     return;
   }
   if (!active) return;
-  const int64_t col = int64_t(x) * g.sx + int64_t(y) * g.sy;
   // interval i of lane L at (tile base) + i * kOutStride + 2 L: the
   // wavefront's i-th intervals are one contiguous run
   const int64_t ob = g.o_base + tile * g.arena + 2 * int64_t(L);
