@@ -134,3 +134,50 @@ def test_batch_program_replays_from_a_graph():
         for d in keep:
             d.free()
         ctx.close()
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_batch_work_queue_equals_flat_grid(prec, monkeypatch):
+    """The persistent work-queue form of the batched passes (VO_BATCH_QUEUE=1:
+    a resident-sized grid taking tiles from per-XCD queues) gives the same
+    results as one workgroup per tile, over more tiles than the grid holds,
+    every radius class, and three calls in a row (eager, captured, replayed)."""
+    from voroffset_amd import _native
+    rng = np.random.default_rng(11)
+    dt = np.float32 if prec == "fp32" else np.float64
+    items = []
+    for k in range(24):
+        side = int(rng.integers(60, 200))
+        r = float([rng.uniform(1.5, 31), rng.uniform(32, 63), rng.uniform(64, 100)][k % 3])
+        vol = synth.poisson_columns(side, side, lam=float(rng.uniform(2, 8)), zmax=400, seed=900 + k,
+                                    z_margin=int(r) + 4)
+        items.append((synth.OPS[k % 4], vol, r))
+    assert sum(v.nx * -(-v.ny // 64) for _, v, _ in items) > 8192  # more tiles than a resident grid
+    keep, jobs = [], []
+    flat = _native.Context(0)
+    monkeypatch.setenv("VO_BATCH_QUEUE", "1")
+    queued = _native.Context(0)
+    monkeypatch.delenv("VO_BATCH_QUEUE")
+    try:
+        for op, vol, r in items:
+            zlo, zhi = vol.z_range()
+            d_off, d_z = _native.DeviceArray(vol.off), _native.DeviceArray(vol.z.astype(dt))
+            keep += [d_off, d_z]
+            jobs.append((op, r, vol.nx, vol.ny, d_off.address, d_z.address, zlo, zhi))
+
+        def results(ctx):
+            outs, _ = ctx.morph_batch_device(prec, jobs)
+            return [(_native.to_host(v.offsets, np.int64, v.nx * v.ny + 1), _native.to_host(v.z, dt, c))
+                    for v, c in outs]
+        want = results(flat)
+        assert flat.last_batch_alone() == 0
+        for call in range(3):
+            got = results(queued)
+            assert queued.last_batch_alone() == 0
+            for k, ((o0, z0), (o1, z1)) in enumerate(zip(want, got)):
+                assert np.array_equal(o0, o1) and np.array_equal(z0, z1), (call, k, items[k][0], items[k][2])
+    finally:
+        for d in keep:
+            d.free()
+        flat.close()
+        queued.close()

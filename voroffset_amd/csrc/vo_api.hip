@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -94,6 +95,7 @@ enum Slot {
   kIN_OFF, kIN_Z,                                                   // staged host input
   kR_V, kR_F, kR_TCNT, kR_TOFF, kR_TCUR, kR_TTRIS, kR_LEN, kR_OFF, kR_Z,  // rasterizer
   kB_TAB0, kB_TAB1, kB_TAB2, kB_STATUS, kB_FIN_OFF, kB_FIN_AUX,         // batched form
+  kB_QUEUE,                                                 // batched form: work-queue counters
   kG_LISTS, kG_HTAB,                                         // gather interval lists, H table
   kM_PAIRS, kM_SEGOFF, kM_VERT, kM_QUAD,                     // dexel mesh writer
   kM_SLO, kM_SHI, kM_CHI,                                    // pass-1 block and column summaries
@@ -166,6 +168,9 @@ struct vo_context {
   size_t batch_host_n = 0;
   BatchGraph bgraph;              // the last batch program, captured on its second call
   bool use_graphs = true;         // VO_BATCH_GRAPH=0: never capture
+  bool batch_queue = false;       // VO_BATCH_QUEUE=1: persistent work-queue grids for the batched passes
+  int num_cus = 0;                // compute units of the device
+  std::map<std::pair<const void*, size_t>, int> resident;  // workgroups per CU, by kernel and dynamic LDS
   int last_fallback_tiles = 0;
   // the radius whose gather table the H-table slot holds, and where (a call
   // with the same radius skips the upload)
@@ -371,31 +376,52 @@ void launch_zsweep_events(vo_context* ctx, int events, int rint, int64_t grid, c
   }
 }
 
-// Batched z-sweep with the same event capacities, over the global tiles
-// [t0, t0 + grid) (one radius class of the batch).
+// Grid of a batched launch over n tiles: one workgroup per tile, or, with a
+// work queue, as many workgroups as the device keeps resident (at most n).
+template <class K>
+int64_t batch_grid(vo_context* ctx, K kernel, size_t lds, int64_t n, const unsigned* queue) {
+  if (!queue) return n;
+  const auto key = std::make_pair(reinterpret_cast<const void*>(kernel), lds);
+  auto it = ctx->resident.find(key);
+  if (it == ctx->resident.end()) {
+    int per_cu = 0;
+    check(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, vo::kWave, lds), "occupancy");
+    it = ctx->resident.emplace(key, std::max(per_cu, 1)).first;
+  }
+  return std::max<int64_t>(1, std::min<int64_t>(n, int64_t(it->second) * std::max(ctx->num_cus, 1)));
+}
+
+// Batched z-sweep with the same event capacities, over the n global tiles
+// from t0 on (one radius class of the batch); queue: 8 cleared counters for
+// the persistent form, or null.
+template <class T, int ECAP, int NB, int NW>
+void launch_zsweep_batch_nw(vo_context* ctx, int64_t n, const vo::Pass1ZArgs<T>* jobs, const int64_t* tile_base,
+                            int njobs, int64_t t0, unsigned* queue) {
+  auto* k = queue ? vo::pass1_zsweep_batch_kernel<T, ECAP, NB, kZWMax, NW, true>
+                  : vo::pass1_zsweep_batch_kernel<T, ECAP, NB, kZWMax, NW, false>;
+  const dim3 g(static_cast<unsigned>(batch_grid(ctx, k, 0, n, queue))), b(static_cast<unsigned>(vo::kWave));
+  hipLaunchKernelGGL(k, g, b, 0, ctx->stream, jobs, tile_base, njobs, t0, n, queue);
+}
+
 template <class T, int ECAP, int NB>
-void launch_zsweep_batch(vo_context* ctx, int max_rint, int64_t grid, const vo::Pass1ZArgs<T>* jobs,
-                         const int64_t* tile_base, int njobs, int64_t t0) {
-  const dim3 g(static_cast<unsigned>(grid)), b(static_cast<unsigned>(vo::kWave));
+void launch_zsweep_batch(vo_context* ctx, int max_rint, int64_t n, const vo::Pass1ZArgs<T>* jobs,
+                         const int64_t* tile_base, int njobs, int64_t t0, unsigned* queue) {
   if (max_rint <= 31)
-    hipLaunchKernelGGL((vo::pass1_zsweep_batch_kernel<T, ECAP, NB, kZWMax, 1>), g, b, 0, ctx->stream, jobs, tile_base,
-                       njobs, t0);
+    launch_zsweep_batch_nw<T, ECAP, NB, 1>(ctx, n, jobs, tile_base, njobs, t0, queue);
   else if (max_rint <= 63)
-    hipLaunchKernelGGL((vo::pass1_zsweep_batch_kernel<T, ECAP, NB, kZWMax, 2>), g, b, 0, ctx->stream, jobs, tile_base,
-                       njobs, t0);
+    launch_zsweep_batch_nw<T, ECAP, NB, 2>(ctx, n, jobs, tile_base, njobs, t0, queue);
   else
-    hipLaunchKernelGGL((vo::pass1_zsweep_batch_kernel<T, ECAP, NB, kZWMax, 4>), g, b, 0, ctx->stream, jobs, tile_base,
-                       njobs, t0);
+    launch_zsweep_batch_nw<T, ECAP, NB, 4>(ctx, n, jobs, tile_base, njobs, t0, queue);
 }
 
 template <class T>
-void launch_zsweep_batch_events(vo_context* ctx, int events, int max_rint, int64_t grid,
-                                const vo::Pass1ZArgs<T>* jobs, const int64_t* tile_base, int njobs, int64_t t0) {
+void launch_zsweep_batch_events(vo_context* ctx, int events, int max_rint, int64_t n, const vo::Pass1ZArgs<T>* jobs,
+                                const int64_t* tile_base, int njobs, int64_t t0, unsigned* queue) {
   switch (events) {
-    case 512: launch_zsweep_batch<T, 512, 256>(ctx, max_rint, grid, jobs, tile_base, njobs, t0); break;
-    case 1024: launch_zsweep_batch<T, 1024, 512>(ctx, max_rint, grid, jobs, tile_base, njobs, t0); break;
-    case 1536: launch_zsweep_batch<T, 1536, 512>(ctx, max_rint, grid, jobs, tile_base, njobs, t0); break;
-    default: launch_zsweep_batch<T, 768, 512>(ctx, max_rint, grid, jobs, tile_base, njobs, t0); break;
+    case 512: launch_zsweep_batch<T, 512, 256>(ctx, max_rint, n, jobs, tile_base, njobs, t0, queue); break;
+    case 1024: launch_zsweep_batch<T, 1024, 512>(ctx, max_rint, n, jobs, tile_base, njobs, t0, queue); break;
+    case 1536: launch_zsweep_batch<T, 1536, 512>(ctx, max_rint, n, jobs, tile_base, njobs, t0, queue); break;
+    default: launch_zsweep_batch<T, 768, 512>(ctx, max_rint, n, jobs, tile_base, njobs, t0, queue); break;
   }
 }
 
@@ -1095,6 +1121,13 @@ void morph_batch_device(vo_context* ctx, int n, const vo_job* jobs, vo_volume* o
   }
   int64_t* len = static_cast<int64_t*>(ws(ctx, kO_LEN, size_t(maxC + 1) * sizeof(int64_t)));
   int* status = static_cast<int*>(ws(ctx, kB_STATUS, size_t(n) * sizeof(int)));
+  // work-queue counters of the persistent form: eight per launch (two passes
+  // and three radius classes per dilation step), cleared when the program starts
+  int ndil = 0;
+  for (auto& st : steps) ndil += st.kind == kStepDilate ? 1 : 0;
+  const int64_t qwords = ctx->batch_queue ? int64_t(ndil) * 2 * 3 * vo::kXcds : 0;
+  unsigned* queue = qwords ? static_cast<unsigned*>(ws(ctx, kB_QUEUE, size_t(qwords) * sizeof(unsigned))) : nullptr;
+  auto queue_of = [&](int di, int pass, int c) { return queue ? queue + ((di * 2 + pass) * 3 + c) * vo::kXcds : nullptr; };
   T *m_lo = nullptr, *m_hi = nullptr, *s_lo = nullptr, *s_hi = nullptr, *c_hi = nullptr, *o_z = nullptr;
   uint16_t* m_dy = nullptr;
   int32_t* m_cnt = nullptr;
@@ -1205,6 +1238,8 @@ void morph_batch_device(vo_context* ctx, int n, const vo_job* jobs, vo_volume* o
   // 5. the program: enqueued, captured or replayed
   auto enqueue = [&] {
     fill_words(ctx, status, n, 0);
+    if (queue) fill_words(ctx, queue, qwords, 0);
+    int di = 0;  // dilation steps so far
     for (auto& st : steps) {
       const int K = int(st.jobs.size());
       const int64_t* dbase = reinterpret_cast<const int64_t*>(dblob + st.t2);
@@ -1216,21 +1251,26 @@ void morph_batch_device(vo_context* ctx, int n, const vo_job* jobs, vo_volume* o
         for (int c = 0; c < 3; ++c)
           if (st.cls_t1[c] > st.cls_t0[c])
             launch_zsweep_batch_events<T>(ctx, events_for(ctx, st.cls_rmax[c]), st.cls_rmax[c],
-                                          st.cls_t1[c] - st.cls_t0[c], d1, dbase, K, st.cls_t0[c]);
+                                          st.cls_t1[c] - st.cls_t0[c], d1, dbase, K, st.cls_t0[c], queue_of(di, 0, c));
         check(ctx, hipGetLastError(), "batched pass1 launch");
         for (int c = 0; c < 3; ++c) {
           if (st.cls_t1[c] <= st.cls_t0[c]) continue;
-          const dim3 grid(static_cast<unsigned>(st.cls_t1[c] - st.cls_t0[c])), block(vo::kWave);
-          if (st.cls_rmax[c] <= vo::kGatherLdsTableMax)
-            hipLaunchKernelGGL((vo::pass2_gather_batch_kernel<T, true>), grid, block,
-                               size_t(st.cls_rmax[c] + 1) * vo::htab_width(st.cls_rmax[c]) * sizeof(double),
-                               ctx->stream, d2,
-                               dbase, K, st.cls_t0[c]);
-          else
-            hipLaunchKernelGGL((vo::pass2_gather_batch_kernel<T, false>), grid, block, 0, ctx->stream, d2, dbase, K,
-                               st.cls_t0[c]);
+          const int64_t nt = st.cls_t1[c] - st.cls_t0[c];
+          unsigned* q = queue_of(di, 1, c);
+          const dim3 block(vo::kWave);
+          if (st.cls_rmax[c] <= vo::kGatherLdsTableMax) {
+            auto* k = q ? vo::pass2_gather_batch_kernel<T, true, true> : vo::pass2_gather_batch_kernel<T, true, false>;
+            const size_t lds = size_t(st.cls_rmax[c] + 1) * vo::htab_width(st.cls_rmax[c]) * sizeof(double);
+            hipLaunchKernelGGL(k, dim3(unsigned(batch_grid(ctx, k, lds, nt, q))), block, lds, ctx->stream, d2, dbase, K,
+                               st.cls_t0[c], nt, q);
+          } else {
+            auto* k = q ? vo::pass2_gather_batch_kernel<T, false, true> : vo::pass2_gather_batch_kernel<T, false, false>;
+            hipLaunchKernelGGL(k, dim3(unsigned(batch_grid(ctx, k, 0, nt, q))), block, 0, ctx->stream, d2, dbase, K,
+                               st.cls_t0[c], nt, q);
+          }
         }
         check(ctx, hipGetLastError(), "batched pass2 launch");
+        ++di;
         exclusive_scan(ctx, len, st.off, st.C + 1);
         if (st.C > 0) {  // (a step over empty grids only has its one offset)
           hipLaunchKernelGGL(vo::gather_kernel<T>, dim3(blocks_for(st.C, 256)), dim3(256), 0, ctx->stream, st.C,
@@ -1276,7 +1316,7 @@ void morph_batch_device(vo_context* ctx, int n, const vo_job* jobs, vo_volume* o
     add_key(st.cls_rmax, sizeof(st.cls_rmax));
     add_key(&st.off, sizeof(st.off));
   }
-  const void* bufs[] = {dblob, hb, len, status, rel, aux, ws(ctx, kSCAN_TMP, 16)};
+  const void* bufs[] = {dblob, hb, len, status, rel, aux, ws(ctx, kSCAN_TMP, 16), queue};
   add_key(bufs, sizeof(bufs));
   for (int c = 0; c < kTuneBuckets; ++c) add_key(&ctx->tuning.events[c], sizeof(int));
 
@@ -1841,6 +1881,7 @@ int vo_create(vo_context** out, int device) {
   if (const char* e = std::getenv("VO_ZSWEEP_EVENTS")) ctx->zsweep_events = std::atoi(e);
   if (const char* e = std::getenv("VO_INJECT_HIP_ERROR")) ctx->inject = e;
   if (const char* e = std::getenv("VO_BATCH_GRAPH")) ctx->use_graphs = std::string(e) != "0";
+  if (const char* e = std::getenv("VO_BATCH_QUEUE")) ctx->batch_queue = std::string(e) == "1";
   if (const char* e = std::getenv("VO_AXIS")) ctx->axis_forced = std::string(e) == "x" ? 1 : (std::string(e) == "y" ? 0 : -1);
   if (const char* e = std::getenv("VO_PASS1"))
     ctx->pass1_form = std::string(e) == "cull" ? 2 : (std::string(e) == "paint" ? 1 : 0);
@@ -1850,6 +1891,7 @@ int vo_create(vo_context** out, int device) {
     if (device < 0 || device >= n) fail(ctx, VO_ERR_ARGUMENT, "no such HIP device");
     check(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "stream");
     for (auto& e : ctx->ev) check(ctx, hipEventCreate(&e), "event");
+    check(ctx, hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount, device), "CU count");
     // coherent: kernels write the control words into it directly
     check(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->host_words), 8 * sizeof(int64_t), hipHostMallocCoherent),
           "pinned");

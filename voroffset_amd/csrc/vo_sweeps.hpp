@@ -1621,15 +1621,26 @@ __global__ __launch_bounds__(kWave, (GatherWaves<T, kLdsTable>::value)) void pas
   gather_tile<T, kLdsTable>(g, xcd_tile(blockIdx.x, gridDim.x));
 }
 
-// Batched: job j owns the global tiles [tile_base[j], tile_base[j + 1]).
-template <class T, bool kLdsTable>
+// Batched: job j owns the global tiles [tile_base[j], tile_base[j + 1]);
+// this launch covers the n tiles from t0 on (one radius class), one per
+// workgroup, or (kQueue) as a persistent grid taking them from the eight
+// counters at `queue` (take_tiles).  The two forms are separate kernels: the
+// tile body inlined in the queue loop needs registers the one-tile form
+// would otherwise spill for.
+template <class T, bool kLdsTable, bool kQueue>
 __global__ __launch_bounds__(kWave, (GatherWaves<T, kLdsTable>::value)) void pass2_gather_batch_kernel(const Pass2GArgs<T>* jobs,
                                                                    const int64_t* tile_base, int njobs,
-                                                                   int64_t t0) {
-  const int64_t t = t0 + xcd_tile(blockIdx.x, gridDim.x);
-  const int j = find_job(tile_base, njobs, t);
-  const Pass2GArgs<T> g = jobs[j];
-  gather_tile<T, kLdsTable>(g, t - tile_base[j]);
+                                                                   int64_t t0, int64_t n, unsigned* queue) {
+  auto body = [&](int64_t i) {
+    const int64_t t = t0 + i;
+    const int j = find_job(tile_base, njobs, t);
+    const Pass2GArgs<T> g = jobs[j];
+    gather_tile<T, kLdsTable>(g, t - tile_base[j]);
+  };
+  if constexpr (kQueue)
+    take_tiles(queue, n, body);
+  else
+    body(xcd_chunk_tile(blockIdx.x, n));
 }
 
 }  // namespace vo
@@ -2753,16 +2764,25 @@ __global__ __launch_bounds__(kWave, zsweep_waves(sizeof(Pass1ZSmem<T, ECAP, NB, 
   zsweep_tile<T, ECAP, NB, WMAX, NW>(g, xcd_tile(blockIdx.x, gridDim.x), sm);
 }
 
-// Batched: job j owns the global tiles [tile_base[j], tile_base[j + 1]).
-template <class T, int ECAP, int NB, int WMAX, int NW>
+// Batched: job j owns the global tiles [tile_base[j], tile_base[j + 1]);
+// this launch covers the n tiles from t0 on (one radius class), one per
+// workgroup, or (kQueue) as a persistent grid (take_tiles), as in
+// pass2_gather_batch_kernel.
+template <class T, int ECAP, int NB, int WMAX, int NW, bool kQueue>
 __global__ __launch_bounds__(kWave, zsweep_waves(sizeof(Pass1ZSmem<T, ECAP, NB, WMAX>))) void pass1_zsweep_batch_kernel(const Pass1ZArgs<T>* jobs,
                                                                    const int64_t* tile_base, int njobs,
-                                                                   int64_t t0) {
+                                                                   int64_t t0, int64_t n, unsigned* queue) {
   __shared__ Pass1ZSmem<T, ECAP, NB, WMAX> sm;
-  const int64_t t = t0 + xcd_tile(blockIdx.x, gridDim.x);
-  const int j = find_job(tile_base, njobs, t);
-  const Pass1ZArgs<T> g = jobs[j];
-  zsweep_tile<T, ECAP, NB, WMAX, NW>(g, t - tile_base[j], sm);
+  auto body = [&](int64_t i) {
+    const int64_t t = t0 + i;
+    const int j = find_job(tile_base, njobs, t);
+    const Pass1ZArgs<T> g = jobs[j];
+    zsweep_tile<T, ECAP, NB, WMAX, NW>(g, t - tile_base[j], sm);
+  };
+  if constexpr (kQueue)
+    take_tiles(queue, n, body);
+  else
+    body(xcd_chunk_tile(blockIdx.x, n));
 }
 
 }  // namespace vo

@@ -41,6 +41,47 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t n) {
   const int64_t xcd = b % kXcds, k = b / kXcds;
   return xcd * q + (xcd < r ? xcd : r) + k;
 }
+// Batched launches, whose tiles differ in cost from volume to volume: the
+// tiles go to the XCDs in chunks of kXcdChunk consecutive tiles, dealt
+// round-robin, so that every XCD gets a share of every stretch of the batch
+// (contiguous eighths left the XCD holding the costliest volumes working
+// alone at the end: 39.5 ms against 35.8 ms for 256 uneven volumes) while
+// neighbouring tiles still meet in one L2.  A bijection on [0, n): the tiles
+// past the last whole round of chunks keep b.
+constexpr int64_t kXcdChunk = 64;
+__device__ __forceinline__ int64_t xcd_chunk_tile(int64_t b, int64_t n) {
+  const int64_t full = n / (kXcds * kXcdChunk) * (kXcds * kXcdChunk);
+  if (b >= full) return b;
+  const int64_t xcd = b % kXcds, k = b / kXcds;
+  return ((k / kXcdChunk) * kXcds + xcd) * kXcdChunk + k % kXcdChunk;
+}
+// Persistent form of a batched launch (one wavefront per workgroup): the
+// grid holds about as many workgroups as the CUs keep resident, and each
+// takes tiles until none is left.  Queue q (q = 0 .. 7, counters cleared
+// before the launch) hands out the tiles xcd_chunk_tile deals to the
+// workgroups b = q, q + 8, ... of a one-tile-per-workgroup grid, i.e. the
+// tiles XCD q would run; a workgroup takes from its own XCD's queue first
+// and then from the others in turn, so an XCD that runs out of work early
+// takes over tiles of the others.  Every tile is taken exactly once, and
+// every wavefront leaves after one failed take per queue.
+template <class F>
+__device__ void take_tiles(unsigned* queue, int64_t n, F&& body) {
+  const int home = int(blockIdx.x % kXcds);
+  for (int s = 0; s < kXcds; ++s) {
+    const int q = (home + s) % kXcds;
+    const int64_t nq = q < n ? (n - 1 - q) / kXcds + 1 : 0;  // tiles b = q, q + 8, ... below n
+    for (;;) {
+      unsigned k = 0;
+      if (lane_id() == 0) k = atomicAdd(&queue[q], 1u);
+      // a scalar: the tile, its job and the job's arguments then stay in
+      // scalar registers, as they do when the tile comes from blockIdx
+      k = __builtin_amdgcn_readfirstlane(k);
+      if (int64_t(k) >= nq) break;
+      body(xcd_chunk_tile(int64_t(k) * kXcds + q, n));
+      __syncthreads();  // the next tile's body reuses the workgroup's LDS
+    }
+  }
+}
 __device__ __forceinline__ void wave_sync() { __syncthreads(); }
 
 // Status bits written by the kernels (host turns them into error codes).
