@@ -162,7 +162,8 @@ def precision_report(items, threads):
     against the FP64 oracle on the FP64 input (so it includes the float32
     rounding of the input), columns compared, columns whose segment counts
     differ, and how many of those only differ by gaps narrower than 1e-5
-    dexel (the rest are listed as unexplained)."""
+    dexel, or by segments that narrow where such a gap is complemented
+    (the rest are listed as unexplained)."""
     import oracle
 
     rep = {}
@@ -185,7 +186,7 @@ def precision_report(items, threads):
                     out[-1] = max(out[-1], z[k + 1])
                 else:
                     out.extend([z[k], z[k + 1]])
-            return len(out)
+            return sum(1 for k in range(0, len(out), 2) if out[k + 1] - out[k] >= 1e-5)
         narrow = sum(1 for c in bad if fused(it["z"][it["off"][c]:it["off"][c + 1]]) ==
                      fused(ref.z[ref.off[c]:ref.off[c + 1]]))
         key = f'{it["config"]}: {it["op"]} r={it["r"]:g}'

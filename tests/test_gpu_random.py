@@ -1,0 +1,115 @@
+"""Seeded random volumes through the HIP path, both storage precisions.
+
+Two families:
+
+* regular columns (no zero-length segments, no touching segments): every
+  operation against the oracle's `ours`, at grid sizes from 1x1 to 96x96
+  (several 64-column tiles), radii 0.3 .. 40 (integer, half-integer and
+  arbitrary), endpoints on a quarter-dexel grid or continuous;
+* degenerate columns (zero-length segments, touching segments) against the
+  oracle's brute force.  On these the reference's `ours` (as restated by the
+  oracle) loses a zero-length segment that lies inside a farther line's
+  segment in pass 1 (its merge keeps the larger-radius pieces by position
+  and drops the point); the HIP path computes the exact morphology, which
+  is what the reference's brute-force method computes
+  (`test_point_inside_farther_segment` below is the smallest such case).
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+from voroffset_amd.volume import CompressedVolume
+
+from .helpers import compare_csr, narrow_gap_mismatches, quantize_fp32
+
+pytestmark = pytest.mark.gpu
+
+OPS = ["dilation", "erosion", "closing", "opening"]
+
+
+def random_case(seed, degenerate):
+    rng = np.random.default_rng(seed)
+    if degenerate:
+        nx, ny = int(rng.integers(1, 33)), int(rng.integers(1, 33))
+        r = float(np.exp(rng.uniform(np.log(0.3), np.log(12))))
+    else:
+        nx, ny = int(rng.integers(1, 97)), int(rng.integers(1, 97))
+        kind = rng.random()
+        r = (float(rng.integers(1, 41)) if kind < 0.3 else float(rng.integers(0, 40)) + 0.5 if kind < 0.5
+             else float(np.exp(rng.uniform(np.log(0.3), np.log(40)))))
+    op = OPS[int(rng.integers(0, 4))]
+    zext = float(rng.integers(150, 500))
+    margin = int(np.ceil(r)) + 3 if op in ("dilation", "closing") else 2
+    lam = float(rng.uniform(0, 10))
+    grid = degenerate or rng.random() < 0.5
+    cols = []
+    for _ in range(nx * ny):
+        v = rng.uniform(margin, zext - margin, 2 * int(rng.poisson(lam)))
+        if grid:
+            v = np.round(v * 4) / 4
+        v.sort()
+        if not degenerate:  # drop zero-length segments, fuse touching ones
+            segs = []
+            for a, b in zip(v[0::2], v[1::2]):
+                if a >= b:
+                    continue
+                if segs and a <= segs[-1][1]:
+                    segs[-1][1] = max(segs[-1][1], b)
+                else:
+                    segs.append([a, b])
+            v = [x for s in segs for x in s]
+        cols.append(list(v))
+    vol = CompressedVolume.from_columns(nx, ny, cols, origin=(0.0, 0.0, 0.0), extent=(float(nx), float(ny), zext),
+                                        spacing=1.0, padding=0)
+    return op, vol, r
+
+
+def _check(hip, prec, op, vol, r, method):
+    src = vol if prec == "fp64" else quantize_fp32(vol)
+    got, _, _ = hip.apply(op, vol, r)
+    ref = oracle.morph(src, op, r, method=method)
+    bad, err = compare_csr(got.off, got.z, *ref)
+    if prec == "fp64":
+        assert len(bad) == 0 and err <= 1e-9, (op, r, vol.nx, vol.ny, bad[:5], err)
+    else:
+        assert narrow_gap_mismatches(got.off, got.z, ref[0], ref[1], bad) and err <= 1e-4, (op, r, bad[:5], err)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_random_regular_columns(prec, hip_fp64, hip_fp32):
+    hip = hip_fp64 if prec == "fp64" else hip_fp32
+    for seed in range(40):
+        op, vol, r = random_case(1000 + seed, degenerate=False)
+        _check(hip, prec, op, vol, r, "ours")
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_random_degenerate_columns(prec, hip_fp64, hip_fp32):
+    hip = hip_fp64 if prec == "fp64" else hip_fp32
+    for seed in range(20):
+        op, vol, r = random_case(2000 + seed, degenerate=True)
+        _check(hip, prec, op, vol, r, "brute_force")
+
+
+def test_point_inside_farther_segment(hip_fp64):
+    """Line x = 1: [203.5, 223.5] at y = 0, a point at 209.25 at y = 1,
+    [197.75, 227.75] at y = 2; R = 1.025.  The point reaches the columns
+    (0, 1) and (2, 1) with half-length sqrt(R^2 - 1); the segments do not
+    (their lines are at distance sqrt(2) > R).  Exact morphology (the HIP
+    path, the oracle's brute force): [209.25 -+ 0.2216] in both columns; the
+    oracle's `ours` drops the point in pass 1 and leaves them empty."""
+    cols = [[] for _ in range(9)]
+    cols[1], cols[4], cols[7] = [203.5, 223.5], [209.25, 209.25], [197.75, 227.75]
+    vol = CompressedVolume.from_columns(3, 3, cols, origin=(0.0, 0.0, 0.0), extent=(3.0, 3.0, 285.0),
+                                        spacing=1.0, padding=0)
+    r = 1.025
+    got, _, _ = hip_fp64.apply("dilation", vol, r)
+    h = np.sqrt(r * r - 1.0)
+    for c in (3, 5):
+        assert np.allclose(got.z[got.off[c]:got.off[c + 1]], [209.25 - h, 209.25 + h], atol=1e-12), c
+    brute = oracle.morph(vol, "dilation", r, method="brute_force")
+    bad, err = compare_csr(got.off, got.z, *brute)
+    assert len(bad) == 0 and err <= 1e-12
+    ours = oracle.morph(vol, "dilation", r)
+    assert ours[0][4] == ours[0][3] and ours[0][6] == ours[0][5]  # columns 3 and 5 empty in `ours`

@@ -54,3 +54,17 @@ def test_csr_valid():
     assert not csr_valid(np.array([0, 3, 4]), np.array([0.0, 1.0, 2.0, 3.0]))   # odd column
     assert not csr_valid(np.array([0, 2, 4]), np.array([1.0, 0.0, 2.0, 3.0]))   # inverted
     assert csr_valid(np.array([0, 2, 4]), np.array([5.0, 6.0, 2.0, 3.0]))       # columns independent
+
+
+def test_narrow_feature_mismatches():
+    from .helpers import narrow_gap_mismatches
+    off1, off2 = np.array([0, 2]), np.array([0, 4])
+    one = np.array([0.0, 3.0])
+    # a gap of 2e-6 in one column and none in the other: explained
+    assert narrow_gap_mismatches(off1, one, off2, np.array([0.0, 1.0, 1.000002, 3.0]), [0])
+    # a segment of width 0 or 1e-7 more in one column: explained
+    assert narrow_gap_mismatches(np.array([0, 4]), np.array([0.0, 3.0, 5.0, 5.0]), off1, one, [0])
+    assert narrow_gap_mismatches(np.array([0, 4]), np.array([0.0, 3.0, 5.0, 5.0000001]), off1, one, [0])
+    # a gap or an extra segment of 1e-3: not explained
+    assert not narrow_gap_mismatches(off1, one, off2, np.array([0.0, 1.0, 1.001, 3.0]), [0])
+    assert not narrow_gap_mismatches(np.array([0, 4]), np.array([0.0, 3.0, 5.0, 5.001]), off1, one, [0])
