@@ -211,7 +211,7 @@ def run_device(ctx, vol, radius, precision, steps, warmup):
     from voroffset_amd import _native
 
     zlo, zhi = vol.z_range()
-    shift = float(np.round(0.5 * (zlo + zhi))) if precision == "fp32" else 0.0
+    shift = _native.storage_shift(zlo, zhi) if precision == "fp32" else 0.0
     dtype = np.float32 if precision == "fp32" else np.float64
     d_off = _native.DeviceArray(vol.off)
     d_z = _native.DeviceArray((vol.z - shift).astype(dtype))
@@ -239,7 +239,7 @@ def tune_context(ctx, lam, radii, side=256, seed=7):
 
     v = synth.poisson_columns(side, side, lam=lam, zmax=1024, seed=seed)
     zlo, zhi = v.z_range()
-    shift = float(np.round(0.5 * (zlo + zhi)))
+    shift = _native.storage_shift(zlo, zhi)
     d_off, d_z = _native.DeviceArray(v.off), _native.DeviceArray((v.z - shift).astype(np.float32))
     try:
         return ctx.autotune("fp32", v.nx, v.ny, d_off.address, d_z.address, radii, reps=3)
@@ -261,7 +261,7 @@ def batched_leg(n_volumes, steps, warmup, seed=4321, tune=True):
     for sp in specs:
         v = synth.poisson_columns(sp["side"], sp["side"], lam=sp["lam"], zmax=1024, seed=sp["seed"])
         zlo, zhi = v.z_range()
-        shift = float(np.round(0.5 * (zlo + zhi)))
+        shift = _native.storage_shift(zlo, zhi)
         d_off = _native.DeviceArray(v.off)
         d32 = _native.DeviceArray((v.z - shift).astype(np.float32))
         d64 = _native.DeviceArray(v.z)
@@ -304,7 +304,7 @@ def batched_leg(n_volumes, steps, warmup, seed=4321, tune=True):
         sp = specs[k]
         src = synth.poisson_columns(sp["side"], sp["side"], lam=sp["lam"], zmax=1024, seed=sp["seed"])
         zlo, zhi = src.z_range()
-        shift = float(np.round(0.5 * (zlo + zhi)))
+        shift = _native.storage_shift(zlo, zhi)
         res = src.like(_native.to_host(v.offsets, np.int64, v.nx * v.ny + 1),
                        _native.to_host(v.z, np.float32, cnt).astype(np.float64) + shift)
         items.append(whole_item(f"batch of {n_volumes}", src, res, sp["op"], sp["radius"]))
@@ -420,7 +420,7 @@ def main():
     tuned = None
     if not args.no_tune:  # untimed: the autotuner picks the shapes for this radius on the workload itself
         zlo, zhi = vol.z_range()
-        shift = float(np.round(0.5 * (zlo + zhi)))
+        shift = _native.storage_shift(zlo, zhi)
         d_off, d_z = _native.DeviceArray(vol.off), _native.DeviceArray((vol.z - shift).astype(np.float32))
         t = time.perf_counter()
         table = ctx.autotune("fp32", vol.nx, vol.ny, d_off.address, d_z.address, [R], reps=3)

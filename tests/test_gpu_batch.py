@@ -9,7 +9,7 @@ import oracle
 from voroffset_amd import synth
 from voroffset_amd.volume import CompressedVolume
 
-from .helpers import compare_csr, narrow_gap_mismatches, quantize_fp32
+from .helpers import compare_csr, narrow_gap_mismatches, quantize_fp32, random_case
 
 pytestmark = pytest.mark.gpu
 
@@ -35,6 +35,21 @@ def test_batch_equals_single_path_fp64(hip_fp64):
         assert np.array_equal(got.z, one.z), op
         bad, err = compare_csr(got.off, got.z, *oracle.morph(vol, op, r))
         assert len(bad) == 0 and err <= 1e-9, (op, r, bad[:5], err)
+
+
+def test_batch_equals_single_path_fp32(hip_fp32):
+    """FP32 storage: the batch quantizes every volume relative to the same
+    integer shift as the host entry point (the z-range centre rounded half
+    away from zero; these volumes have odd z-extents, where rounding half to
+    even would differ), so batched results equal the single-volume ones bit
+    for bit."""
+    items = [random_case(3000 + k, degenerate=False) for k in range(24)]
+    assert any(float(v.extent[2]) % 2 == 1 for _, v, _ in items)
+    res, _, _ = hip_fp32.apply_batch(items)
+    for (op, vol, r), got in zip(items, res):
+        one, _, _ = hip_fp32.apply(op, vol, r)
+        assert np.array_equal(got.off, one.off), (op, r)
+        assert np.array_equal(got.z, one.z), (op, r)
 
 
 def test_batch_large_radii_fp64(hip_fp64):

@@ -21,49 +21,9 @@ import pytest
 import oracle
 from voroffset_amd.volume import CompressedVolume
 
-from .helpers import compare_csr, narrow_gap_mismatches, quantize_fp32
+from .helpers import compare_csr, narrow_gap_mismatches, quantize_fp32, random_case
 
 pytestmark = pytest.mark.gpu
-
-OPS = ["dilation", "erosion", "closing", "opening"]
-
-
-def random_case(seed, degenerate):
-    rng = np.random.default_rng(seed)
-    if degenerate:
-        nx, ny = int(rng.integers(1, 33)), int(rng.integers(1, 33))
-        r = float(np.exp(rng.uniform(np.log(0.3), np.log(12))))
-    else:
-        nx, ny = int(rng.integers(1, 97)), int(rng.integers(1, 97))
-        kind = rng.random()
-        r = (float(rng.integers(1, 41)) if kind < 0.3 else float(rng.integers(0, 40)) + 0.5 if kind < 0.5
-             else float(np.exp(rng.uniform(np.log(0.3), np.log(40)))))
-    op = OPS[int(rng.integers(0, 4))]
-    zext = float(rng.integers(150, 500))
-    margin = int(np.ceil(r)) + 3 if op in ("dilation", "closing") else 2
-    lam = float(rng.uniform(0, 10))
-    grid = degenerate or rng.random() < 0.5
-    cols = []
-    for _ in range(nx * ny):
-        v = rng.uniform(margin, zext - margin, 2 * int(rng.poisson(lam)))
-        if grid:
-            v = np.round(v * 4) / 4
-        v.sort()
-        if not degenerate:  # drop zero-length segments, fuse touching ones
-            segs = []
-            for a, b in zip(v[0::2], v[1::2]):
-                if a >= b:
-                    continue
-                if segs and a <= segs[-1][1]:
-                    segs[-1][1] = max(segs[-1][1], b)
-                else:
-                    segs.append([a, b])
-            v = [x for s in segs for x in s]
-        cols.append(list(v))
-    vol = CompressedVolume.from_columns(nx, ny, cols, origin=(0.0, 0.0, 0.0), extent=(float(nx), float(ny), zext),
-                                        spacing=1.0, padding=0)
-    return op, vol, r
-
 
 def _check(hip, prec, op, vol, r, method):
     src = vol if prec == "fp64" else quantize_fp32(vol)

@@ -28,7 +28,7 @@ def test_noalloc_equals_device_entry_point(prec, op):
         for k, (nx, ny, r) in enumerate([(96, 80, 7.5), (41, 37, 3.0)]):   # the second call is smaller
             vol = synth.poisson_columns(nx, ny, lam=5, zmax=300, seed=50 + k, z_margin=20)
             zlo, zhi = vol.z_range()
-            shift = float(np.round(0.5 * (zlo + zhi))) if prec == "fp32" else 0.0
+            shift = _native.storage_shift(zlo, zhi) if prec == "fp32" else 0.0
             d_off, d_z = _native.DeviceArray(vol.off), _native.DeviceArray((vol.z - shift).astype(dt))
             keep += [d_off, d_z]
             ref_off, ref_z = _device_result(ctx, prec, op, r, vol, d_off, d_z, zlo - shift, zhi - shift)

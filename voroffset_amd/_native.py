@@ -18,6 +18,15 @@ OK, ERR_ARGUMENT, ERR_DEVICE, ERR_CAPACITY, ERR_RANGE = 0, 1, 2, 3, 4
 OP_CODES = {"noop": 0, "dilation": 1, "erosion": 2, "closing": 3, "opening": 4}
 PRECISIONS = {"fp32": 0, "fp64": 1}
 
+
+def storage_shift(z_lo, z_hi):
+    """The integer z-shift FP32 endpoints are stored relative to: the centre
+    of [z_lo, z_hi] rounded half away from zero, as the host entry point
+    rounds it (std::round), so that every entry point quantizes a volume the
+    same way."""
+    c = 0.5 * (float(z_lo) + float(z_hi))
+    return float(np.copysign(np.floor(abs(c) + 0.5), c))
+
 # every symbol the header declares
 EXPORTS = ("vo_create", "vo_destroy", "vo_last_error", "vo_morph", "vo_result_shape", "vo_result_copy",
            "vo_result_free", "vo_morph_device", "vo_last_stats", "vo_rasterize", "vo_morph_batch_device",

@@ -54,7 +54,7 @@ class VoronoiMorphoHip:
         try:
             for op, vol, r in items:
                 zlo, zhi = vol.z_range()
-                shift = float(np.round(0.5 * (zlo + zhi))) if self.precision == "fp32" else 0.0
+                shift = _native.storage_shift(zlo, zhi) if self.precision == "fp32" else 0.0
                 d_off = _native.DeviceArray(vol.off)
                 d_z = _native.DeviceArray((vol.z - shift).astype(dtype))
                 keep += [d_off, d_z]
@@ -81,7 +81,7 @@ class VoronoiMorphoHip:
         numSegments (CompressedVolume.cpp:61-83), reduced on the device."""
         import numpy as np
         zlo, zhi = volume.z_range()
-        shift = float(np.round(0.5 * (zlo + zhi))) if self.precision == "fp32" else 0.0  # lengths do not see it
+        shift = _native.storage_shift(zlo, zhi) if self.precision == "fp32" else 0.0  # lengths do not see it
         d_off, d_z = self._upload(volume, shift)
         try:
             return self.ctx.measure_device(self.precision, volume.nx, volume.ny, d_off.address, d_z.address,
@@ -96,7 +96,7 @@ class VoronoiMorphoHip:
         import numpy as np
         dtype = np.float32 if self.precision == "fp32" else np.float64
         zlo, zhi = a.z_range()
-        shift = float(np.round(0.5 * (zlo + zhi))) if self.precision == "fp32" else 0.0
+        shift = _native.storage_shift(zlo, zhi) if self.precision == "fp32" else 0.0
         arrays = [*self._upload(a, shift), *self._upload(b, shift)]
         try:
             va = _native.VoVolume(a.nx, a.ny, arrays[0].address, arrays[1].address, zlo - shift, zhi - shift)

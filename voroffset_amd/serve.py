@@ -129,7 +129,7 @@ class MorphoServer:
     def put(self, vol):
         """Uploads a CompressedVolume; returns its handle."""
         zlo, zhi = vol.z_range()
-        shift = float(np.round(0.5 * (zlo + zhi))) if self.precision == "fp32" else 0.0
+        shift = _native.storage_shift(zlo, zhi) if self.precision == "fp32" else 0.0
         dtype = np.float32 if self.precision == "fp32" else np.float64
         r = Resident(vol.nx, vol.ny, _native.DeviceArray(vol.off), _native.DeviceArray((vol.z - shift).astype(dtype)),
                      zlo - shift, zhi - shift, shift, vol.spacing, int(vol.z.size))
