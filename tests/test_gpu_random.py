@@ -73,3 +73,33 @@ def test_point_inside_farther_segment(hip_fp64):
     assert len(bad) == 0 and err <= 1e-12
     ours = oracle.morph(vol, "dilation", r)
     assert ours[0][4] == ours[0][3] and ours[0][6] == ours[0][5]  # columns 3 and 5 empty in `ours`
+
+
+def _large_radius_volume(op, r):
+    from voroffset_amd import synth
+    if op == "dilation":
+        return synth.poisson_columns(37, 45, lam=5, zmax=900, seed=int(r), z_margin=int(r) + 4)
+    if op == "closing":  # the erosion step empties everything within R of the grid's border
+        return synth.poisson_columns(280, 270, lam=1, zmax=900, seed=7, z_margin=int(r) + 4)
+    n = 330  # erosion of a box: columns farther than R from the border keep [100 + R, 900 - R]
+    return CompressedVolume.from_columns(n, n, [[100.0, 900.0]] * (n * n), origin=(0.0, 0.0, 0.0),
+                                         extent=(float(n), float(n), 1000.0), spacing=1.0, padding=0)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("op,r", [("dilation", 140.5), ("dilation", 300.0), ("closing", 131.0), ("erosion", 150.0)])
+def test_radii_beyond_the_z_sweep(prec, op, r, hip_fp64, hip_fp32):
+    """R > 127: pass 1 runs as the painter form, pass 2 still as the gather
+    (whose lists live in global memory; the sweep form's LDS lists overflowed
+    on such volumes)."""
+    hip = hip_fp64 if prec == "fp64" else hip_fp32
+    vol = _large_radius_volume(op, r)
+    src = vol if prec == "fp64" else quantize_fp32(vol)
+    got, _, _ = hip.apply(op, vol, r)
+    ref = oracle.morph(src, op, r, threads=8)
+    assert ref[0][-1] > 0
+    bad, err = compare_csr(got.off, got.z, *ref)
+    if prec == "fp64":
+        assert len(bad) == 0 and err <= 1e-9, (op, r, bad[:5], err)
+    else:
+        assert narrow_gap_mismatches(got.off, got.z, ref[0], ref[1], bad) and err <= 1e-4, (op, r, bad[:5], err)

@@ -53,7 +53,7 @@ struct Tuning {
 };
 // pass-1 z-sweep form: LDS events per z-slab (denser windows take more slabs)
 // and histogram bins; small enough for ~5 waves per SIMD
-constexpr int kZECap = 768, kZNB = 512, kZWMax = 64 + 2 * 128;
+constexpr int kZECap = 768, kZNB = 512, kZWMax = 64 + 2 * 128, kZWMax8 = 64 + 2 * 256;
 
 template <class T>
 struct Shapes;
@@ -354,7 +354,8 @@ const double* upload_htab(vo_context* ctx, int slot, const std::vector<double>& 
 }
 
 // The z-sweep form of pass 1 with 2R + 2 coverage bits per lane (one 64-bit
-// word up to R = 31, two up to 63, four up to 127).
+// word up to R = 31, two up to 63, four up to 127, eight up to 255).
+constexpr int kZSweepMaxR = 255;
 template <class T, int ECAP, int NB>
 void launch_zsweep(vo_context* ctx, int rint, int64_t grid, const vo::Pass1ZArgs<T>& az) {
   const dim3 g(static_cast<unsigned>(grid)), b(static_cast<unsigned>(vo::kWave));
@@ -362,8 +363,10 @@ void launch_zsweep(vo_context* ctx, int rint, int64_t grid, const vo::Pass1ZArgs
     hipLaunchKernelGGL((vo::pass1_zsweep_kernel<T, ECAP, NB, kZWMax, 1>), g, b, 0, ctx->stream, az);
   else if (rint <= 63)
     hipLaunchKernelGGL((vo::pass1_zsweep_kernel<T, ECAP, NB, kZWMax, 2>), g, b, 0, ctx->stream, az);
-  else
+  else if (rint <= 127)
     hipLaunchKernelGGL((vo::pass1_zsweep_kernel<T, ECAP, NB, kZWMax, 4>), g, b, 0, ctx->stream, az);
+  else
+    hipLaunchKernelGGL((vo::pass1_zsweep_kernel<T, ECAP, NB, kZWMax8, 8>), g, b, 0, ctx->stream, az);
 }
 
 template <class T>
@@ -519,7 +522,7 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
     // pass 2: gather tiles of (x, 64 consecutive y), and the sweep form,
     // either over the whole volume in chunks of chunk2 columns of a row, or
     // (list mode) over the gather tiles that overflowed, one column a block
-    const bool gather = rint <= 128 && !ctx->force_sweep;
+    const bool gather = rint <= vo::kGatherMaxR && !ctx->force_sweep;
     const int64_t ntiles = int64_t(nx) * ((ny + vo::kGatherTile - 1) / vo::kGatherTile);
     int gocap = ocap_for(ctx, rint);
     const int chunk2_eff = gather ? 1 : chunk2;
@@ -535,7 +538,7 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
     int* tiles = static_cast<int*>(ws(ctx, kTILES, (ntiles + 1) * sizeof(int)));
     if (grid1 > INT32_MAX || ntiles > INT32_MAX || sweep_blocks > INT32_MAX)
       fail(ctx, VO_ERR_ARGUMENT, "grid too large");
-    const bool zsweep = ctx->pass1_form == 0 && rint <= 127 && !zsweep_failed &&
+    const bool zsweep = ctx->pass1_form == 0 && rint <= kZSweepMaxR && !zsweep_failed &&
                         vo::zsweep_offsets_fit(caps, ny, int(sizeof(T)));
     if (chk_init && !zsweep) {  // the check's own kernel, words and copy
       vo::InputCheck<T> c{in.off, static_cast<const T*>(in.z), chk_range, chk_lo, chk_hi};

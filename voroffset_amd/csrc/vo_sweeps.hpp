@@ -1227,6 +1227,11 @@ constexpr int gather_group(bool small_r) { return small_r ? 1 : 4; }
 // 8.25 KB); above it the lanes read the table from global memory
 // (L1-resident).
 constexpr int kGatherLdsTableMax = 31;
+// Largest radius the gather form takes (its H table is 33.6 MB there); above
+// it pass 2 runs as the sweep form, whose LDS lists bound the density it
+// takes.  The gather itself keeps its lists in global memory and has no such
+// bound (a tile whose lists outgrow their room goes to the sweep form).
+constexpr int kGatherMaxR = 2048;
 
 // The gather's H table: row |d| holds H(|d|, dy) = sqrt(R^2 - d^2 - dy^2)
 // for dy = 0 .. rint, NaN where that is negative, and one more NaN entry
