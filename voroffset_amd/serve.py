@@ -232,12 +232,15 @@ class MorphoServer:
 def simulate(server, handles, rate, n_requests, radius_range=(2.0, 48.0), seed=0, saturate=False):
     """Poisson arrivals at `rate` per second of n requests over the resident
     `handles` (op uniform over the four operations, radius uniform); runs the
-    server in real time and returns the finished requests.  With `saturate`
-    the same requests all arrive at once (the server's capacity)."""
+    server in real time and returns the finished requests.  The arrival
+    times are a Poisson process conditioned on n arrivals in n / rate
+    seconds (n sorted uniform times), so the offered rate is `rate` itself
+    rather than one draw's.  With `saturate` the same requests all arrive at
+    once (the server's capacity)."""
     rng = np.random.default_rng(seed)
-    gaps = rng.exponential(1.0 / rate, n_requests)
+    times = np.sort(rng.uniform(0.0, n_requests / rate, n_requests))
     t0 = time.perf_counter() + 0.01
-    arrivals = t0 + (np.zeros(n_requests) if saturate else np.cumsum(gaps))
+    arrivals = t0 + (np.zeros(n_requests) if saturate else times)
     plan = [(int(rng.integers(len(handles))), OPS[int(rng.integers(4))], float(rng.uniform(*radius_range)))
             for _ in range(n_requests)]
     done, k = [], 0
