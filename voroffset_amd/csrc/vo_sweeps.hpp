@@ -2279,7 +2279,7 @@ struct ZState {
 // bit i of the lanes in the run through an XOR difference array (the bits
 // are distinct, so OR = XOR) and a prefix XOR.  Then every lane walks its
 // own events in order, as the lane walk does.
-constexpr int kCellMaxR = 63;
+constexpr int kCellMaxR = 127;
 
 template <class T, int MW>
 struct CellWalk {
@@ -2298,7 +2298,8 @@ struct CellWalk {
     for (int t = 0; t < MW; ++t) carry[t] = 0;
   }
 
-  static constexpr int kNone = 0xff;
+  // an event word: c, p + 1, q + 1 in 9 bits each (window columns < 511), start flag above
+  static constexpr int kNone = 0x1ff;
 
   // sweep the E events (ez, ep) of one z-slab; `diff` is LDS scratch of
   // kWave + 1 words
@@ -2370,8 +2371,8 @@ struct CellWalk {
       }
       // the event as one word for the lanes that read it: c, p + 1, q + 1
       // (kNone beyond the window), start flag
-      const int pk = c | ((p >= 0 ? p + 1 : 0) << 8) | ((q < (1 << 20) ? q + 1 : kNone) << 16) |
-                     (start ? 1 << 24 : 0);
+      const int pk = c | ((p >= 0 ? p + 1 : 0) << 9) | ((q < (1 << 20) ? q + 1 : kNone) << 18) |
+                     (start ? 1 << 27 : 0);
       const int pos = L + R;
       for (;;) {
         const bool on = mine != 0;
@@ -2380,9 +2381,9 @@ struct CellWalk {
         mine &= mine - 1;
         const T z = __shfl(vz, i);
         const int e = __shfl(pk, i);
-        const int ec = e & 0xff, ep1 = (e >> 8) & 0xff, eq1 = (e >> 16) & 0xff;
+        const int ec = e & kNone, ep1 = (e >> 9) & kNone, eq1 = (e >> 18) & kNone;
         int dn;
-        if (e >> 24) {
+        if (e >> 27) {
           dn = abs(ec - pos);
         } else {
           const int dp = ep1 ? pos - (ep1 - 1) : 1 << 20;
@@ -2526,9 +2527,9 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
         raw_check(chk && w >= R && w < R + kZTile && y0 + (w - R) < g.ny, prev));
   }
   report_check();
-  // windows without zero-length segments take the cell walk (R <= 63; the
-  // lane state for R <= 31 has room for 128 window columns)
-  const bool cells = NW <= 2 && ballot(points != 0) == 0 && R <= (NW == 1 ? 31 : kCellMaxR);
+  // windows without zero-length segments take the cell walk (R <= 127: its
+  // covered mask has 2, 3 or 5 words for the window's 64 + 2R columns)
+  const bool cells = NW <= 4 && ballot(points != 0) == 0 && R <= kCellMaxR;
   wave_sync();
   // slabs of consecutive bins holding at most ECAP endpoints each: one slab
   // when everything fits (the usual case), else a greedy split
@@ -2619,7 +2620,7 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
   stage.ny = uint32_t(g.ny);
   stage.caps = g.caps;
   // covered-mask words of the cell walk: the window's 64 + 2R columns
-  CellWalk<T, NW == 1 ? 2 : 3> cw(R, active);
+  CellWalk<T, NW == 1 ? 2 : (NW == 2 ? 3 : 5)> cw(R, active);
   int b_lo = 0;
   for (int sl = 0; sl < nslab; ++sl) {
     const int b_hi = sm.slab_end[sl];
@@ -2697,7 +2698,7 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
     wave_sync();
     // Sweep in (z, kind) order: the cell walk when it applies, else the lane
     // walk below.
-    if constexpr (NW <= 2) {
+    if constexpr (NW <= 4) {
       if (cells) {
         cw.sweep(sm.ez, sm.ep, E, sm.colmask, g, stage);
         b_lo = b_hi;
