@@ -24,6 +24,7 @@
 #include <mutex>
 #include <string>
 #include <type_traits>
+#include <thread>
 #include <vector>
 
 #include "../../include/voroffset_hip.h"
@@ -166,6 +167,8 @@ struct vo_context {
   ScratchArena arena;        // on during vo_morph_device_noalloc
   int64_t* batch_host = nullptr;  // pinned: a batch program's counts and statuses
   size_t batch_host_n = 0;
+  void* staging = nullptr;        // pinned: host volumes on their way to and from the device
+  size_t staging_bytes = 0;
   BatchGraph bgraph;              // the last batch program, captured on its second call
   bool use_graphs = true;         // VO_BATCH_GRAPH=0: never capture
   bool batch_queue = false;       // VO_BATCH_QUEUE=1: persistent work-queue grids for the batched passes
@@ -1634,6 +1637,31 @@ void validate_shape(vo_context* ctx, const vo_volume* v) {
 }
 
 // Host-side checks of the CompressedVolume invariants the sweeps rely on.
+// f(begin, end) over [0, n) on up to 16 host threads (the calling thread
+// takes the first chunk); small ranges run on the calling thread alone.
+template <class F>
+void host_parallel(int64_t n, F&& f) {
+  const int64_t kMinChunk = int64_t(1) << 16;
+  const int hw = int(std::max(1u, std::thread::hardware_concurrency()));
+  const int nt = int(std::min<int64_t>(std::min(hw, 16), std::max<int64_t>(1, n / kMinChunk)));
+  if (nt <= 1) {
+    f(int64_t(0), n);
+    return;
+  }
+  std::vector<std::thread> pool;
+  const int64_t step = (n + nt - 1) / nt;
+  for (int t = 1; t < nt; ++t) {
+    const int64_t b = std::min(n, t * step), e = std::min(n, b + step);
+    pool.emplace_back([&f, b, e] { f(b, e); });
+  }
+  f(int64_t(0), std::min(n, step));
+  for (auto& th : pool) th.join();
+}
+
+// The host entry point's input checks.  Offsets first (column order, so the
+// endpoints are read only within them), then the endpoints of every column
+// in parallel; the lowest failing column is reported, with the first of its
+// failures in endpoint order.
 void validate_host(vo_context* ctx, const vo_volume* v, int op, double radius) {
   const int64_t n = int64_t(v->nx) * v->ny;
   const double* z = static_cast<const double*>(v->z);
@@ -1641,17 +1669,43 @@ void validate_host(vo_context* ctx, const vo_volume* v, int op, double radius) {
   for (int64_t c = 0; c < n; ++c) {
     const int64_t s = v->offsets[c], e = v->offsets[c + 1];
     if (e < s || (e - s) % 2) fail(ctx, VO_ERR_ARGUMENT, "column " + std::to_string(c) + " has odd length");
-    for (int64_t i = s; i < e; ++i) {
-      if (!std::isfinite(z[i])) fail(ctx, VO_ERR_ARGUMENT, "non-finite endpoint");
-      if (i > s && z[i] < z[i - 1]) fail(ctx, VO_ERR_ARGUMENT, "column " + std::to_string(c) + " is unsorted");
-      if (op == VO_OP_EROSION || op == VO_OP_OPENING) {
-        if (z[i] < v->z_lo - 1 || z[i] > v->z_hi + 1)
-          fail(ctx, VO_ERR_RANGE, "erosion input endpoint outside [z_lo - 1, z_hi + 1]");
-      }
-      if (op == VO_OP_CLOSING && (z[i] - radius < v->z_lo - 1 || z[i] + radius > v->z_hi + 1))
-        fail(ctx, VO_ERR_RANGE, "closing: the dilated volume would leave [z_lo - 1, z_hi + 1]");
-    }
   }
+  std::mutex mu;
+  int64_t bad_col = n;
+  int bad_code = VO_OK;
+  std::string bad_text;
+  host_parallel(n, [&](int64_t c0, int64_t c1) {
+    for (int64_t c = c0; c < c1; ++c) {
+      const int64_t s = v->offsets[c], e = v->offsets[c + 1];
+      int code = VO_OK;
+      std::string text;
+      for (int64_t i = s; i < e && code == VO_OK; ++i) {
+        if (!std::isfinite(z[i])) {
+          code = VO_ERR_ARGUMENT;
+          text = "non-finite endpoint";
+        } else if (i > s && z[i] < z[i - 1]) {
+          code = VO_ERR_ARGUMENT;
+          text = "column " + std::to_string(c) + " is unsorted";
+        } else if ((op == VO_OP_EROSION || op == VO_OP_OPENING) && (z[i] < v->z_lo - 1 || z[i] > v->z_hi + 1)) {
+          code = VO_ERR_RANGE;
+          text = "erosion input endpoint outside [z_lo - 1, z_hi + 1]";
+        } else if (op == VO_OP_CLOSING && (z[i] - radius < v->z_lo - 1 || z[i] + radius > v->z_hi + 1)) {
+          code = VO_ERR_RANGE;
+          text = "closing: the dilated volume would leave [z_lo - 1, z_hi + 1]";
+        }
+      }
+      if (code != VO_OK) {
+        std::lock_guard<std::mutex> lock(mu);
+        if (c < bad_col) {
+          bad_col = c;
+          bad_code = code;
+          bad_text = text;
+        }
+        return;  // the rest of this chunk is above c
+      }
+    }
+  });
+  if (bad_code != VO_OK) fail(ctx, bad_code, bad_text);
 }
 
 // Mesh -> dexels (create_dexels' ray casting, Dexelize.cpp:166-225); see
@@ -1728,20 +1782,45 @@ double storage_shift(const vo_volume* in) {
   return std::is_same<T, float>::value ? std::round(0.5 * (in->z_lo + in->z_hi)) : 0.0;
 }
 
+// The context's pinned staging buffer, at least `bytes` long (grown with
+// the stream drained: no copy from the old buffer is in flight).
+void* staging(vo_context* ctx, size_t bytes) {
+  if (bytes > ctx->staging_bytes) {
+    const size_t want = std::max(bytes, ctx->staging_bytes + ctx->staging_bytes / 2);
+    check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    if (ctx->staging) check(ctx, hipHostFree(ctx->staging), "free staging");
+    ctx->staging = nullptr;
+    ctx->staging_bytes = 0;
+    check(ctx, hipHostMalloc(&ctx->staging, want, hipHostMallocDefault), "pinned staging");
+    ctx->staging_bytes = want;
+  }
+  return ctx->staging;
+}
+
 // A checked host volume (double z) uploaded in storage precision T, minus
-// `shift`.  `zt` holds the converted endpoints until the upload has run.
+// `shift`, through the staging buffer (which the call's later
+// synchronisation frees for reuse).
 template <class T>
-DevVol upload_host(vo_context* ctx, const vo_volume* in, double shift, std::vector<T>& zt) {
+DevVol upload_host(vo_context* ctx, const vo_volume* in, double shift) {
   const int64_t n = int64_t(in->nx) * in->ny;
   const int64_t cnt = n ? in->offsets[n] : 0;
-  zt.assign(size_t(std::max<int64_t>(cnt, 1)), T(0));
+  const size_t zn = size_t(std::max<int64_t>(cnt, 1));
+  // offsets and converted endpoints go through the pinned staging buffer
+  char* st = static_cast<char*>(staging(ctx, (n + 1) * sizeof(int64_t) + zn * sizeof(T)));
+  int64_t* so = reinterpret_cast<int64_t*>(st);
+  T* sz = reinterpret_cast<T*>(st + (n + 1) * sizeof(int64_t));
   const double* z = static_cast<const double*>(in->z);
-  for (int64_t i = 0; i < cnt; ++i) zt[i] = T(z[i] - shift);
+  host_parallel(n + 1, [&](int64_t b, int64_t e) {
+    std::memcpy(so + b, in->offsets + b, size_t(e - b) * sizeof(int64_t));
+  });
+  host_parallel(cnt, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) sz[i] = T(z[i] - shift);
+  });
+  if (cnt == 0) sz[0] = T(0);
   int64_t* doff = static_cast<int64_t*>(ws(ctx, kIN_OFF, (n + 1) * sizeof(int64_t)));
-  T* dz = static_cast<T*>(ws(ctx, kIN_Z, zt.size() * sizeof(T)));
-  check(ctx, hipMemcpyAsync(doff, in->offsets, (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream),
-        "upload");
-  check(ctx, hipMemcpyAsync(dz, zt.data(), zt.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
+  T* dz = static_cast<T*>(ws(ctx, kIN_Z, zn * sizeof(T)));
+  check(ctx, hipMemcpyAsync(doff, so, (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream), "upload");
+  check(ctx, hipMemcpyAsync(dz, sz, zn * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
   DevVol v;
   v.nx = in->nx;
   v.ny = in->ny;
@@ -1757,8 +1836,7 @@ void morph_host(vo_context* ctx, int op, double R, const vo_volume* in, vo_resul
   validate_shape(ctx, in);
   validate_host(ctx, in, op, R);
   const double shift = storage_shift<T>(in);
-  std::vector<T> zt;
-  DevVol v = upload_host<T>(ctx, in, shift, zt);
+  DevVol v = upload_host<T>(ctx, in, shift);
   DevVol r = v;
   if (op != VO_OP_NOOP) r = run_op<T>(ctx, op, v, R, in->z_lo - shift, in->z_hi - shift, t1, t2);
   auto* res = new vo_result;
@@ -1766,13 +1844,21 @@ void morph_host(vo_context* ctx, int op, double R, const vo_volume* in, vo_resul
   res->ny = r.ny;
   const int64_t rn = int64_t(r.nx) * r.ny;
   res->off.resize(rn + 1);
-  std::vector<T> rz(size_t(std::max<int64_t>(r.count, 1)));
-  check(ctx, hipMemcpyAsync(res->off.data(), r.off, (rn + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream),
-        "download");
-  check(ctx, hipMemcpyAsync(rz.data(), r.z, rz.size() * sizeof(T), hipMemcpyDeviceToHost, ctx->stream), "download");
+  // through the staging buffer: the upload from it ran before these copies
+  const size_t zn = size_t(std::max<int64_t>(r.count, 1));
+  char* st = static_cast<char*>(staging(ctx, (rn + 1) * sizeof(int64_t) + zn * sizeof(T)));
+  int64_t* so = reinterpret_cast<int64_t*>(st);
+  T* sz = reinterpret_cast<T*>(st + (rn + 1) * sizeof(int64_t));
+  check(ctx, hipMemcpyAsync(so, r.off, (rn + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream), "download");
+  check(ctx, hipMemcpyAsync(sz, r.z, zn * sizeof(T), hipMemcpyDeviceToHost, ctx->stream), "download");
   check(ctx, hipStreamSynchronize(ctx->stream), "sync");
   res->z.resize(r.count);
-  for (int64_t i = 0; i < r.count; ++i) res->z[i] = double(rz[i]) + shift;
+  host_parallel(rn + 1, [&](int64_t b, int64_t e) {
+    std::memcpy(res->off.data() + b, so + b, size_t(e - b) * sizeof(int64_t));
+  });
+  host_parallel(r.count, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) res->z[i] = double(sz[i]) + shift;
+  });
   *out = res;
 }
 
@@ -1784,8 +1870,7 @@ void pass1_host(vo_context* ctx, double R, const vo_volume* in, vo_triples* res)
   validate_shape(ctx, in);
   validate_host(ctx, in, VO_OP_DILATION, R);
   const double shift = storage_shift<T>(in);
-  std::vector<T> zt;
-  DevVol v = upload_host<T>(ctx, in, shift, zt);
+  DevVol v = upload_host<T>(ctx, in, shift);
   // the reference's axis: lines of constant x, swept along y
   const int forced = ctx->axis_forced;
   ctx->axis_forced = 0;
@@ -1930,6 +2015,7 @@ void vo_destroy(vo_context* ctx) {
     if (e) (void)hipEventDestroy(e);
   if (ctx->host_words) (void)hipHostFree(ctx->host_words);
   if (ctx->batch_host) (void)hipHostFree(ctx->batch_host);
+  if (ctx->staging) (void)hipHostFree(ctx->staging);
   if (ctx->bgraph.exec) (void)hipGraphExecDestroy(ctx->bgraph.exec);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
