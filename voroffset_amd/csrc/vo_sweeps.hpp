@@ -2279,7 +2279,7 @@ struct ZState {
 // bit i of the lanes in the run through an XOR difference array (the bits
 // are distinct, so OR = XOR) and a prefix XOR.  Then every lane walks its
 // own events in order, as the lane walk does.
-constexpr int kCellMaxR = 127;
+constexpr int kCellMaxR = 255;
 
 template <class T, int MW>
 struct CellWalk {
@@ -2298,8 +2298,8 @@ struct CellWalk {
     for (int t = 0; t < MW; ++t) carry[t] = 0;
   }
 
-  // an event word: c, p + 1, q + 1 in 9 bits each (window columns < 511), start flag above
-  static constexpr int kNone = 0x1ff;
+  // an event word: c, p + 1, q + 1 in 10 bits each (window columns < 1023), start flag above
+  static constexpr int kBits = 10, kNone = (1 << kBits) - 1;
 
   // sweep the E events (ez, ep) of one z-slab; `diff` is LDS scratch of
   // kWave + 1 words
@@ -2371,8 +2371,8 @@ struct CellWalk {
       }
       // the event as one word for the lanes that read it: c, p + 1, q + 1
       // (kNone beyond the window), start flag
-      const int pk = c | ((p >= 0 ? p + 1 : 0) << 9) | ((q < (1 << 20) ? q + 1 : kNone) << 18) |
-                     (start ? 1 << 27 : 0);
+      const int pk = c | ((p >= 0 ? p + 1 : 0) << kBits) | ((q < (1 << 20) ? q + 1 : kNone) << (2 * kBits)) |
+                     (start ? 1 << (3 * kBits) : 0);
       const int pos = L + R;
       for (;;) {
         const bool on = mine != 0;
@@ -2381,9 +2381,9 @@ struct CellWalk {
         mine &= mine - 1;
         const T z = __shfl(vz, i);
         const int e = __shfl(pk, i);
-        const int ec = e & kNone, ep1 = (e >> 9) & kNone, eq1 = (e >> 18) & kNone;
+        const int ec = e & kNone, ep1 = (e >> kBits) & kNone, eq1 = (e >> (2 * kBits)) & kNone;
         int dn;
-        if (e >> 27) {
+        if (e >> (3 * kBits)) {
           dn = abs(ec - pos);
         } else {
           const int dp = ep1 ? pos - (ep1 - 1) : 1 << 20;
@@ -2527,9 +2527,9 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
         raw_check(chk && w >= R && w < R + kZTile && y0 + (w - R) < g.ny, prev));
   }
   report_check();
-  // windows without zero-length segments take the cell walk (R <= 127: its
-  // covered mask has 2, 3 or 5 words for the window's 64 + 2R columns)
-  const bool cells = NW <= 4 && ballot(points != 0) == 0 && R <= kCellMaxR;
+  // windows without zero-length segments take the cell walk (R <= 255: its
+  // covered mask has 2, 3, 5 or 9 words for the window's 64 + 2R columns)
+  const bool cells = ballot(points != 0) == 0 && R <= kCellMaxR;
   wave_sync();
   // slabs of consecutive bins holding at most ECAP endpoints each: one slab
   // when everything fits (the usual case), else a greedy split
@@ -2620,7 +2620,7 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
   stage.ny = uint32_t(g.ny);
   stage.caps = g.caps;
   // covered-mask words of the cell walk: the window's 64 + 2R columns
-  CellWalk<T, NW == 1 ? 2 : (NW == 2 ? 3 : 5)> cw(R, active);
+  CellWalk<T, NW == 1 ? 2 : (NW == 2 ? 3 : (NW == 4 ? 5 : 9))> cw(R, active);
   int b_lo = 0;
   for (int sl = 0; sl < nslab; ++sl) {
     const int b_hi = sm.slab_end[sl];
@@ -2698,12 +2698,10 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
     wave_sync();
     // Sweep in (z, kind) order: the cell walk when it applies, else the lane
     // walk below.
-    if constexpr (NW <= 4) {
-      if (cells) {
-        cw.sweep(sm.ez, sm.ep, E, sm.colmask, g, stage);
-        b_lo = b_hi;
-        continue;
-      }
+    if (cells) {
+      cw.sweep(sm.ez, sm.ep, E, sm.colmask, g, stage);
+      b_lo = b_hi;
+      continue;
     }
     // Lane walk: events are read 64 at a time into registers; a lane's
     // events are those of the 2R + 1 window columns within its reach
