@@ -248,6 +248,46 @@ def tune_context(ctx, lam, radii, side=256, seed=7):
         d_z.free()
 
 
+def tune_batch(ctx, jobs, buckets, choices=(512, 768, 1024, 1536), reps=2):
+    """Untimed: tunes the z-sweep events per slab of each radius bucket the
+    batch uses on the batch itself (one bucket at a time, the others held),
+    keeping the default where no choice is faster; returns the tuning."""
+    import tempfile
+    from voroffset_amd import _native
+
+    def run():
+        ctx.morph_batch_device("fp32", jobs)  # eager, then captured: timed replays
+        ctx.morph_batch_device("fp32", jobs)
+        _native.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            ctx.morph_batch_device("fp32", jobs)
+        _native.synchronize()
+        return (time.perf_counter() - t) / reps
+
+    table = ctx.tuning()
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "tuning")
+
+        def load(tab):
+            with open(path, "w") as f:
+                f.write("voroffset_amd tuning 1\n" + "".join(f"{b} {e} {o}\n" for b, (e, o) in enumerate(tab)))
+            ctx.tuning_load(path)
+        best = run()
+        for b in buckets:
+            for e in choices:
+                if e == table[b][0]:
+                    continue
+                trial = list(table)
+                trial[b] = (e, table[b][1])
+                load(trial)
+                t = run()
+                if t < best * 0.99:
+                    best, table = t, trial
+        load(table)
+    return table
+
+
 def batched_leg(n_volumes, steps, warmup, seed=4321, tune=True):
     """The uneven batch: n volumes, side log-uniform in [64, 512], Poisson(lambda
     ~ U[2, 10]) segments per column, r ~ U[2, 32], ops uniform over dilation /
@@ -270,8 +310,11 @@ def batched_leg(n_volumes, steps, warmup, seed=4321, tune=True):
         jobs64.append((sp["op"], sp["radius"], v.nx, v.ny, d_off.address, d64.address, zlo, zhi))
         cols += v.ncols
     ctx = _native.Context(0)
+    tuned = None
     if tune:
-        tune_context(ctx, 6, [4.0, 12.0, 24.0, 32.0])
+        buckets = sorted({_bucket(sp["radius"]) for sp in specs})
+        tuned = tune_batch(ctx, jobs32, buckets)
+        log(f"batch tuning (events per slab, list capacity per radius bucket): {tuned}")
     for _ in range(warmup):
         ctx.morph_batch_device("fp32", jobs32)
     _native.synchronize()
@@ -317,7 +360,8 @@ def batched_leg(n_volumes, steps, warmup, seed=4321, tune=True):
                         f"r U[2,32], ops uniform over dilation/erosion/closing/opening, seed {seed}",
             "columns": cols, "batched_fp32": {"value": round(cols / tb, 1), "ms_per_batch": round(tb * 1e3, 3)},
             "plain_fp64": {"value": round(cols / tp, 1), "ms_per_batch": round(tp * 1e3, 3)},
-            "batched_over_plain": round(tp / tb, 3), "volumes_redone_alone": alone}, items
+            "batched_over_plain": round(tp / tb, 3), "volumes_redone_alone": alone,
+            "tuning": tuned}, items
 
 
 def served_leg(rate, n_requests, pool, seed=99, tune=True):
@@ -443,14 +487,15 @@ def main():
         plain = {"value": round(ncols / f64_s, 1), "ms_per_step": round(f64_s * 1e3, 4),
                  "pass1_ms": round(f64_p1, 4), "pass2_ms": round(f64_p2, 4)}
 
-    # the mixed legs keep the default shapes: tuned on one sample they ran
-    # slower on the uneven volumes (measured: batch 110 ms tuned, 89 ms not)
+    # the batch is tuned on its own uneven volumes (shapes tuned on one
+    # uniform sample made it slower: 39.3 ms against 35.6 ms untuned); the
+    # served leg keeps the default shapes
     items = []
     batched = None
     if args.batch_volumes > 0:
         # two untimed batches: the first runs eagerly, the second captures the
         # program into a HIP graph; the timed ones replay it (steady state)
-        batched, more = batched_leg(args.batch_volumes, max(2, args.steps // 5), 2, tune=False)
+        batched, more = batched_leg(args.batch_volumes, max(2, args.steps // 5), 2, tune=not args.no_tune)
         items += more
 
     served = None
