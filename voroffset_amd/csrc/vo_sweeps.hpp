@@ -2441,8 +2441,12 @@ template <class T, int ECAP, int NB, int WMAX, int NW>
 __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, ECAP, NB, WMAX>& sm) {
   static_assert(NB <= 32 * kWave, "one bit per lane's bin chunk in a 32-bit mask");
   const int L = lane_id();
-  const int x = int(tile % g.nx);
-  const int y0 = int(tile / g.nx) * kZTile;
+  // tiles of one line are consecutive: neighbouring tiles share 2R columns
+  // of their windows, and xcd_tile hands them to one XCD, whose L2 then
+  // serves the overlap (and the window's second and third reads)
+  const int64_t nyt = (g.ny + kZTile - 1) / kZTile;
+  const int x = int(tile / nyt);
+  const int y0 = int(tile % nyt) * kZTile;
   if (y0 >= g.ny) return;
   const int R = g.rint;
   const int wlo = y0 - R, wn = kZTile + 2 * R;
