@@ -103,3 +103,23 @@ def test_radii_beyond_the_z_sweep(prec, op, r, hip_fp64, hip_fp32):
         assert len(bad) == 0 and err <= 1e-9, (op, r, bad[:5], err)
     else:
         assert narrow_gap_mismatches(got.off, got.z, ref[0], ref[1], bad) and err <= 1e-4, (op, r, bad[:5], err)
+
+
+def test_repeated_dense_calls_keep_their_workspace_bounded(hip_fp64):
+    """Dense windows at large radii make pass 1 retry with more room per
+    column; the room grows only where a form overflowed, so a context that
+    runs many such calls in a row keeps a bounded workspace (the sweep
+    form's pass-2 arena once grew with every retry and ran the device out of
+    memory after a few calls)."""
+    from voroffset_amd import synth
+    from .helpers import csr_valid
+    last = None
+    for lam in (6, 12, 20, 30):
+        for r in (260.0, 300.0, 400.0):
+            vol = synth.poisson_columns(48, 40, lam=lam, zmax=1500, seed=int(lam * 10 + r), z_margin=int(r) + 4)
+            got, _, _ = hip_fp64.apply("dilation", vol, r)
+            assert csr_valid(got.off, got.z), (lam, r)
+            last = (vol, r, got)
+    vol, r, got = last
+    bad, err = compare_csr(got.off, got.z, *oracle.morph(vol, "dilation", r, threads=8))
+    assert len(bad) == 0 and err <= 1e-9, (bad[:5], err)

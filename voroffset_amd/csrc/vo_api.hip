@@ -280,8 +280,15 @@ void* ws(vo_context* ctx, int slot, size_t bytes) {
     }
     b.ptr = nullptr;
     const size_t want = bytes + bytes / 4;
-    check(ctx, hipMalloc(&b.ptr, want), "hipMalloc");
+    if (hipMalloc(&b.ptr, want) != hipSuccess) {
+      b.ptr = nullptr;
+      b.bytes = 0;
+      (void)hipGetLastError();
+      fail(ctx, VO_ERR_DEVICE, "hipMalloc of " + std::to_string(want) + " bytes (workspace slot " +
+                                   std::to_string(slot) + "): out of memory");
+    }
     b.bytes = want;
+    check(ctx, hipSuccess, "hipMalloc");  // (the fault hook's label)
   }
   return b.ptr;
 }
@@ -313,6 +320,8 @@ std::string status_text(int s) {
   if (s & vo::kStatusP2Interiors) t += " pass-2 interior list;";
   if (s & vo::kStatusP2Disks) t += " pass-2 disk list;";
   if (s & vo::kStatusP2Slice) t += " pass-2 line slice;";
+  if (s & vo::kStatusSlotOverflow) t += " pass-1 slots of a column;";
+  if (s & vo::kStatusP2Arena) t += " pass-2 output arena of a column;";
   return "a sweep exceeded its LDS capacity:" + t;
 }
 
@@ -511,6 +520,9 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
   int64_t fast_total = -1;     // the output size, when the gather's one synchronisation settled it
   int64_t gather_end = 0;      // the gather form's part of the output arena (interleaved columns)
   bool copied = false;         // the output CSR was copied before the gather's synchronisation
+  // the per-column capacities as this call found them: a call that gives up
+  // leaves them so, so that failing calls do not compound their growth
+  const int caps_at_entry = ctx->p1_caps, p2cap_at_entry = ctx->p2_cap_per_col;
   for (int attempt = 0;; ++attempt) {
     const int nch1 = (ny + chunk1 - 1) / chunk1;
     const int64_t grid1 = int64_t(nx) * nch1;
@@ -681,14 +693,21 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
       continue;
     }
     if (st & vo::kStatusListOverflow) fail(ctx, VO_ERR_CAPACITY, status_text(st));
-    // arena overflow: give each column more room (LDS arenas: shorter chunks)
-    if (attempt >= 6) fail(ctx, VO_ERR_CAPACITY, "arena overflow after retries");
-    // (the pass-1 forward arena in LDS wants shorter chunks, the slot
-    // layouts and the pass-2 arena want more room per column)
-    if (chunk1 > 8) chunk1 /= 2;
-    if (chunk2 > 8) chunk2 /= 2;
-    ctx->p1_caps *= 2;
-    ctx->p2_cap_per_col *= 2;
+    // an arena overflowed: more room for what overflowed, and again
+    if (attempt >= 6) {
+      ctx->p1_caps = caps_at_entry;
+      ctx->p2_cap_per_col = p2cap_at_entry;
+      fail(ctx, VO_ERR_CAPACITY, "arena overflow after retries");
+    }
+    // (the sweep forms' LDS arenas want shorter chunks, the pass-1 slot
+    // layout more slots per column, the sweep form of pass 2 a larger
+    // output arena per column; the gather sizes its own)
+    if (st & vo::kStatusArenaOverflow) {
+      if (chunk1 > 8) chunk1 /= 2;
+      if (chunk2 > 8) chunk2 /= 2;
+    }
+    if (st & vo::kStatusSlotOverflow) ctx->p1_caps *= 2;
+    if (st & vo::kStatusP2Arena) ctx->p2_cap_per_col *= 2;
   }
   // output CSR (scanned already when the gather settled it)
   int64_t total = fast_total;

@@ -258,7 +258,7 @@ __global__ __launch_bounds__(kWave) void pass1_kernel(Pass1Args<T> g) {
     MergeScratch<T> ms{sm.ev, sm.ec, sm.tlo, sm.thi, sm.tcode, S::ECAP};
     const int nu = merge_max(A, fn, B, n, U, 2 * CAP, ms);
     if (nu < 0 || nu > g.caps) {
-      raise_status(g.status, nu < 0 ? kStatusP1Union : kStatusArenaOverflow);
+      raise_status(g.status, nu < 0 ? kStatusP1Union : kStatusSlotOverflow);
       return;
     }
     for (int i = L; i < nu; i += kWave) {
@@ -420,7 +420,7 @@ __device__ int pass2_step(S& sm, int& cs, int& ns, int& cp, int& np, int k, int 
   }
   // insert the column's triples, a batch at a time
   if (mcnt < 0 || mcnt > g.caps) {
-    raise_status(g.status, kStatusArenaOverflow);
+    raise_status(g.status, kStatusSlotOverflow);
     return -1;
   }
   for (int b0 = 0; b0 < mcnt; b0 += ncap) {
@@ -722,7 +722,7 @@ __global__ __launch_bounds__(kWave) void pass2_kernel(Pass2Args<T> g) {
     const int nr = union_lists(sm.flo + fa, sm.fhi + fa, fn, sm.ulo[2], sm.uhi[2], nu, sm.mlo, sm.mhi,
                                2 * OCAP, sm.ulo[0], sm.uhi[0], OCAP);
     if (nr < 0 || opos + 2 * nr > g.arena) {
-      raise_status(g.status, nr < 0 ? kStatusP2Slice : kStatusArenaOverflow);
+      raise_status(g.status, nr < 0 ? kStatusP2Slice : kStatusP2Arena);
       return;
     }
     for (int i = L; i < nr; i += kWave) {
@@ -1905,7 +1905,7 @@ __global__ __launch_bounds__(kWave) void pass1_paint_kernel(Pass1Args<T> g) {
     MergeScratch<T> ms{sm.ev, sm.ec, sm.tlo, sm.thi, sm.tcode, S::ECAP};
     const int nu = merge_max(A, fn, B, n, U, 2 * CAP, ms);
     if (nu < 0 || nu > g.caps) {
-      raise_status(g.status, nu < 0 ? kStatusP1Union : kStatusArenaOverflow);
+      raise_status(g.status, nu < 0 ? kStatusP1Union : kStatusSlotOverflow);
       return;
     }
     for (int i = L; i < nu; i += kWave) {
@@ -2753,7 +2753,7 @@ __device__ void zsweep_tile(const Pass1ZArgs<T>& g, int64_t tile, Pass1ZSmem<T, 
     st.ok = cw.ok;
     st.last_hi = cw.last_hi;
   }
-  if (ballot(active && st.ok == 0) != 0) raise_status(g.status, kStatusArenaOverflow);
+  if (ballot(active && st.ok == 0) != 0) raise_status(g.status, kStatusSlotOverflow);
   if (active) g.m_cnt[count_index(g.ny, x, y0 + L)] = st.count;
   if (active && st.count > 0) g.c_hi[count_index(g.ny, x, y0 + L)] = st.last_hi;
   stage.finish(st.count, st.last_hi, active);

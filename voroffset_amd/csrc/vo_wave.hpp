@@ -86,13 +86,15 @@ __device__ __forceinline__ void wave_sync() { __syncthreads(); }
 
 // Status bits written by the kernels (host turns them into error codes).
 enum : int {
-  kStatusArenaOverflow = 1,   // an output arena (LDS or HBM) exceeded its capacity
+  kStatusArenaOverflow = 1,   // an LDS arena of a sweep form exceeded its capacity (shorter chunks fit)
   kStatusP1List = 2,          // pass 1: live piece list
   kStatusP1Union = 4,         // pass 1: forward/backward union
   kStatusP2Interiors = 8,     // pass 2: interior list
   kStatusP2Disks = 16,        // pass 2: disk list
   kStatusP2Slice = 32,        // pass 2: intervals of one line slice
   kStatusGatherOverflow = 64, // pass 2 gather form, batched: a tile's lists outgrew LDS
+  kStatusSlotOverflow = 128,  // pass 1: a column's triples exceeded its slots (caps)
+  kStatusP2Arena = 256,       // pass 2 sweep form: a column's output exceeded its arena
   kStatusListOverflow = kStatusP1List | kStatusP1Union | kStatusP2Interiors | kStatusP2Disks | kStatusP2Slice,
 };
 
