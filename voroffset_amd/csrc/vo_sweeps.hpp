@@ -1294,10 +1294,6 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
     const int m = min(g.m_cnt[count_index(g.ny, c, yc)], g.caps);
     return active ? m : 0;
   };
-  int jmax = 0;
-  for (int c = c_lo; c <= c_hi; ++c) jmax = max(jmax, count_of(c));
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) jmax = max(jmax, __shfl_xor(jmax, d));
   int n = 0;
   // loop-carried lane flags are 0/1 integers: in vector registers they need
   // no exec-mask merges at the joins of the loops
@@ -1572,6 +1568,10 @@ __device__ void gather_tile(const Pass2GArgs<T>& g, int64_t tile) {
     // Without summaries: blocks outermost, source columns nearest first
     // within a block, so a block's cache lines are read by all its steps
     // at about the same time.
+    int jmax = 0;
+    for (int c = c_lo; c <= c_hi; ++c) jmax = max(jmax, count_of(c));
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) jmax = max(jmax, __shfl_xor(jmax, d));
     for (int jb = 0; jb < jmax; jb += B) {
       for (int step = 0; step <= 2 * rint; ++step) {
         const int ad = (step + 1) >> 1;
