@@ -137,6 +137,30 @@ def test_dense_column_fp32(hip_fp32, r):
     check_fp32((res.off, res.z), oracle.morph(vol, "dilation", r))
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_dense_column_painter_large_lists(prec, monkeypatch):
+    """The painter form of pass 1 (the form above R = 255; forced here) keeps
+    its piece lists in LDS: a column of 600 segments overflows the default
+    lists, and the dilation reruns pass 1 with the large-list form (one
+    wavefront per CU) instead of failing.  (At R = 40 every column of this
+    5 x 5 grid reaches the 600 segments, and the painter's per-chunk output
+    arena in LDS is the limit: the z-sweep form, the default up to R = 255,
+    takes that case, test_dense_column_fp64.)"""
+    from voroffset_amd.morpho import VoronoiMorphoHip
+    monkeypatch.setenv("VO_PASS1", "paint")
+    hip = VoronoiMorphoHip(prec)
+    monkeypatch.delenv("VO_PASS1")
+    try:
+        vol = _dense_column_volume()
+        src = vol if prec == "fp64" else quantize_fp32(vol)
+        for r in (0.3,):
+            res, _, _ = hip.dilation(src, r)
+            ref = oracle.morph(src, "dilation", r)
+            (check_fp64 if prec == "fp64" else check_fp32)((res.off, res.z), ref)
+    finally:
+        hip.close()
+
+
 def test_empty_grid(hip_fp64):
     vol = _vol(0, 0, [])
     res, _, _ = hip_fp64.dilation(vol, 2.0)

@@ -58,13 +58,17 @@ constexpr int kZECap = 768, kZNB = 512, kZWMax = 64 + 2 * 128, kZWMax8 = 64 + 2 
 
 template <class T>
 struct Shapes;
+// (p1_cap_big: the painter's piece lists when the default ones overflow,
+// as long as its LDS fits a CU's 160 KB)
 template <>
 struct Shapes<float> {
+  static constexpr int p1_cap_big = 1024;
   static constexpr int p1_arena = 2048, p1_cmax = 64;
   static constexpr int p2_farena = 512, p2_cmax = 64;
 };
 template <>
 struct Shapes<double> {
+  static constexpr int p1_cap_big = 640;
   static constexpr int p1_arena = 1024, p1_cmax = 32;
   static constexpr int p2_farena = 256, p2_cmax = 32;
 };
@@ -79,6 +83,10 @@ template <class T>
 using P1P = vo::Pass1PSmem<T, kP1Cap, Shapes<T>::p1_arena, Shapes<T>::p1_cmax>;
 static_assert(sizeof(P1P<float>) <= 65536, "pass-1 painter LDS (fp32)");
 static_assert(sizeof(P1P<double>) <= 65536, "pass-1 painter LDS (fp64)");
+template <class T>
+using P1PBig = vo::Pass1PSmem<T, Shapes<T>::p1_cap_big, Shapes<T>::p1_arena, Shapes<T>::p1_cmax>;
+static_assert(sizeof(P1PBig<float>) <= 160 * 1024, "pass-1 painter LDS, large lists (fp32)");
+static_assert(sizeof(P1PBig<double>) <= 160 * 1024, "pass-1 painter LDS, large lists (fp64)");
 static_assert(sizeof(P2S<float>) <= 65536, "pass-2 LDS (fp32)");
 static_assert(sizeof(P2S<double>) <= 65536, "pass-2 LDS (fp64)");
 static_assert(sizeof(vo::Pass1ZSmem<double, kZECap, kZNB, kZWMax>) <= 65536, "pass-1 z-sweep LDS");
@@ -517,6 +525,7 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
   int chunk1 = std::min(Shapes<T>::p1_cmax, 32);
   int chunk2 = Shapes<T>::p2_cmax;
   bool zsweep_failed = false;  // a window too dense for the z-sweep form: sweep along y instead
+  bool paint_big = false;      // the painter's piece lists overflowed: its large-list form
   int64_t fast_total = -1;     // the output size, when the gather's one synchronisation settled it
   int64_t gather_end = 0;      // the gather form's part of the output arena (interleaved columns)
   bool copied = false;         // the output CSR was copied before the gather's synchronisation
@@ -583,6 +592,9 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
       launch_zsweep_events<T>(ctx, events_for(ctx, rint), rint, gridz, az);
     } else if (ctx->pass1_form == 2) {
       hipLaunchKernelGGL((vo::pass1_kernel<T, kP1Cap, kP1NCap, Shapes<T>::p1_arena, Shapes<T>::p1_cmax>),
+                         dim3(unsigned(grid1)), dim3(vo::kWave), 0, ctx->stream, a1);
+    } else if (paint_big) {
+      hipLaunchKernelGGL((vo::pass1_paint_kernel<T, Shapes<T>::p1_cap_big, Shapes<T>::p1_arena, Shapes<T>::p1_cmax>),
                          dim3(unsigned(grid1)), dim3(vo::kWave), 0, ctx->stream, a1);
     } else {
       hipLaunchKernelGGL((vo::pass1_paint_kernel<T, kP1Cap, Shapes<T>::p1_arena, Shapes<T>::p1_cmax>),
@@ -688,8 +700,12 @@ DevVol dilate(vo_context* ctx, const DevVol& in, double R, int out_off_slot, int
       }
       break;
     }
-    if ((st & vo::kStatusP1List) && ctx->pass1_form == 0 && !zsweep_failed) {
+    if ((st & vo::kStatusP1List) && ctx->pass1_form == 0 && !zsweep_failed && zsweep) {
       zsweep_failed = true;
+      continue;
+    }
+    if ((st & vo::kStatusP1List) && !zsweep && ctx->pass1_form != 2 && !paint_big) {
+      paint_big = true;  // one wavefront per CU, for the rare windows that need it
       continue;
     }
     if (st & vo::kStatusListOverflow) fail(ctx, VO_ERR_CAPACITY, status_text(st));
